@@ -1,0 +1,188 @@
+/*
+ * kagome_vmc.h -- C ABI of the MI355X (gfx950) Gutzwiller-projected
+ * determinant VMC sampler for the Kagome Heisenberg model.
+ *
+ * One handle holds a batch of independent walkers (Markov chains) on one GPU:
+ * per walker the configurations kappa_up/kappa_down and the matrices
+ * W_sigma = U_sigma * tildeU_sigma^-1.  The entry points replace the
+ * reference's Carlo.jl plugin hooks on KagomeDSL.MC (hz-xiaxz/KagomeDSL.jl);
+ * each declaration names the reference function it stands in for.
+ *
+ * Conventions (same as the reference's Julia data):
+ *   - sites and labels are 1-based; kappa[s] = 0 marks an empty site;
+ *   - U matrices are column-major ns x N (Julia Matrix layout), FP64;
+ *   - per-walker arrays are walker-major: walker w's kappa is
+ *     kappa[w*ns .. w*ns+ns-1];
+ *   - bonds are 1-based (i, j) pairs with i <= j in get_nn order
+ *     (src/Hamiltonian.jl:447-451).
+ * Host pointers are borrowed for the duration of a call.  All work runs on
+ * the handle's HIP stream; calls that return host data synchronise.
+ * Every call returns KVMC_OK (0) or an error code; kvmc_last_error() gives
+ * the message for the calling thread.
+ */
+#ifndef KAGOME_VMC_H
+#define KAGOME_VMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KVMC_OK 0
+#define KVMC_ERR_INVALID 1   /* bad argument (ArgumentError / DimensionMismatch) */
+#define KVMC_ERR_SINGULAR 2  /* singular tildeU (SingularException) */
+#define KVMC_ERR_HIP 3       /* HIP runtime failure */
+#define KVMC_ERR_NOMEM 4     /* device allocation failed */
+#define KVMC_ERR_NOT_READY 5 /* W not initialised (set configs first) */
+
+/* W storage precision. FP64: ratios, rank-1 updates and W in double.
+ * FP32: W, ratios and rank-1 (or delayed rank-k) updates in float; local
+ * energies accumulate in double; refresh LU and solve in double. */
+#define KVMC_PREC_FP64 0
+#define KVMC_PREC_FP32 1
+
+/* Transition kernel.  REFERENCE: the reference's shared-uniform kernel
+ * (src/MonteCarlo.jl:538-607), stationary law |psi|^2 / Z_mu.
+ * UNBIASED: uniform bond, accept with min(1, |ratio|^2), law |psi|^2. */
+#define KVMC_KERNEL_REFERENCE 0
+#define KVMC_KERNEL_UNBIASED 1
+
+/* Refresh policy. REFERENCE: after a sweep that passed both early returns
+ * when sweeps % min(N_up, N_down) == 0 (src/MonteCarlo.jl:593-604).
+ * EVERY_K: after every refresh_k accepted moves of a walker. */
+#define KVMC_REFRESH_REFERENCE 0
+#define KVMC_REFRESH_EVERY_K 1
+
+/* Per-sweep outcome codes written by kvmc_sweep (optional record). */
+#define KVMC_OUT_PRE_REJECT 0 /* r > Z_mu / Z_max (src/MonteCarlo.jl:547-550) */
+#define KVMC_OUT_NO_MOVE 1    /* drawn bond not antiparallel (:564-567) */
+#define KVMC_OUT_REJECT 2     /* Metropolis reject */
+#define KVMC_OUT_ACCEPT 3     /* accepted move (:581-591) */
+
+typedef struct kvmc_handle kvmc_handle;
+
+typedef struct kvmc_desc {
+    int32_t ns;             /* sites */
+    int32_t n_up, n_down;   /* particles per spin (columns of U) */
+    int32_t nb;             /* bonds */
+    const int32_t* bonds;   /* 2*nb, 1-based, get_nn order */
+    const double* u_up;     /* ns x n_up column-major */
+    const double* u_down;   /* ns x n_down column-major */
+    int64_t nwalkers;
+    int32_t precision;      /* KVMC_PREC_* */
+    int32_t kernel;         /* KVMC_KERNEL_* */
+    int32_t refresh_policy; /* KVMC_REFRESH_* */
+    int32_t refresh_k;      /* accepted moves per refresh for EVERY_K */
+    int32_t delay;          /* FP32 only: pending rank-1 updates kept before one
+                               rank-`delay` flush of W (<= 1: immediate) */
+    int32_t threads;        /* threads per workgroup of the sweep kernel (0: auto) */
+    uint64_t seed;          /* Philox key of the device RNG */
+} kvmc_desc;
+
+typedef struct kvmc_counters {
+    int64_t sweeps;      /* walker-sweeps executed */
+    int64_t moves;       /* proposals that reached the ratio (passed both early returns) */
+    int64_t accepted;    /* accepted moves */
+    int64_t refreshes;   /* walker refreshes (both spins) */
+    int64_t flushes;     /* delayed-update flushes */
+    int64_t launches;    /* sweep-kernel launches */
+} kvmc_counters;
+
+/* MC(params) + the W/kappa allocation of the low-level constructor
+ * (src/MonteCarlo.jl:165-185, 209-234).  Copies U and the bonds to the
+ * device and allocates per-walker state for desc->nwalkers walkers. */
+int kvmc_create(const kvmc_desc* desc, kvmc_handle** out);
+
+/* Frees all device memory of the handle. */
+int kvmc_destroy(kvmc_handle* h);
+
+/* Sets kappa for all walkers (nwalkers*ns each), zeroes pending updates and
+ * recomputes W by refresh (find_initial_configuration! / read_checkpoint!,
+ * src/MonteCarlo.jl:382-410, 751-755; unlike the reference, W is rebuilt on
+ * restore).  Returns KVMC_ERR_SINGULAR if any tildeU is singular; the first
+ * failing walker is reported by kvmc_last_error and kvmc_failed_walker. */
+int kvmc_set_configs(kvmc_handle* h, const int32_t* kappa_up, const int32_t* kappa_down);
+
+/* Copies kappa of all walkers to the host (write_checkpoint, :715-719). */
+int kvmc_get_configs(kvmc_handle* h, int32_t* kappa_up, int32_t* kappa_down);
+
+/* Seeded uniform Sz=0 start for every walker (replaces the QR start of
+ * Carlo.init!, src/MonteCarlo.jl:435-441): Philox Fisher-Yates permutation
+ * keyed (seed, walker, attempt); walkers whose tildeU is singular are
+ * redrawn up to max_attempts times.  *redraws receives the number of redraws. */
+int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_t* redraws);
+
+/* reevaluateW! for every walker (src/MonteCarlo.jl:55-66): FP64 pivoted LU
+ * of tildeU and W = U * tildeU^-1, converted to the storage precision. */
+int kvmc_refresh(kvmc_handle* h);
+
+/* Carlo.sweep! (src/MonteCarlo.jl:538-607), nsweeps times on every walker,
+ * including the refreshes the policy triggers.  Draws: Philox keyed
+ * (desc.seed, walker, sweep counter) when tape_u is NULL, otherwise the tape
+ * (nwalkers*nsweeps entries each, walker-major: u in [0,1) and 0-based bond
+ * index).  outcomes (optional, nwalkers*nsweeps int8) receives KVMC_OUT_*
+ * per sweep.  counters (optional) receives this call's totals. */
+int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
+               int8_t* outcomes, kvmc_counters* counters);
+
+/* getOL for every walker (src/Hamiltonian.jl:762-778): Heisenberg J1 local
+ * energy real(<x|H|psi>/<x|psi>), accumulated in FP64.  eloc: nwalkers
+ * doubles (may be NULL); sum (optional) receives sum over walkers and
+ * n_antiparallel (optional) the total antiparallel-bond count. */
+int kvmc_measure(kvmc_handle* h, double* eloc, double* sum, int64_t* n_antiparallel);
+
+/* Sets ns x N column-major derivatives dU_k/dtheta_k of the occupied
+ * orbitals for nparams variational parameters (nparams*ns*N doubles per
+ * spin, parameter-major).  Enables kvmc_measure_grad.  No reference
+ * counterpart: the reference has no differentiation. */
+int kvmc_set_orbital_derivs(kvmc_handle* h, int32_t nparams, const double* du_up,
+                            const double* du_down);
+
+/* Local energy and log-derivatives O_k(x) = d log det tildeU_up/dtheta_k +
+ * d log det tildeU_dn/dtheta_k for every walker.  eloc: nwalkers doubles;
+ * o: nwalkers*nparams doubles (walker-major). */
+int kvmc_measure_grad(kvmc_handle* h, double* eloc, double* o);
+
+/* log|det tildeU_up| + log|det tildeU_dn| per walker from the last refresh. */
+int kvmc_get_logdet(kvmc_handle* h, double* logdet);
+
+/* Copies W_sigma (spin 0 = up, 1 = down) of one walker as FP64, column-major
+ * ns x N, with pending delayed updates applied. */
+int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out);
+
+/* Sets / reads ctx.sweeps of every walker (Carlo's sweep counter). */
+int kvmc_set_sweep_count(kvmc_handle* h, int64_t sweeps);
+int kvmc_get_sweep_count(kvmc_handle* h, int64_t* sweeps);
+
+/* Largest max|W - W_refreshed| / max|W_refreshed| over walkers, measured by
+ * recomputing W in FP64 without replacing it (drift of the rank-1 updates). */
+int kvmc_w_drift(kvmc_handle* h, double* max_rel_drift);
+
+/* Kernel timing measured with HIP events on the handle's stream (no
+ * reference counterpart; used by benchmarks).  Enabling resets the sums. */
+typedef struct kvmc_profile {
+    double sweep_ms;         /* sweep-kernel time */
+    double refresh_ms;       /* refresh-kernel time */
+    double measure_ms;       /* local-energy kernel time */
+    int64_t sweep_launches;
+    int64_t refresh_launches;
+    int64_t measure_launches;
+} kvmc_profile;
+int kvmc_profile_enable(kvmc_handle* h, int32_t enable);
+int kvmc_get_profile(kvmc_handle* h, kvmc_profile* out);
+
+/* Walker index of the last singular refresh (-1 if none). */
+int64_t kvmc_failed_walker(kvmc_handle* h);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* kvmc_last_error(void);
+
+/* Library version string. */
+const char* kvmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KAGOME_VMC_H */

@@ -1,0 +1,721 @@
+// C ABI of the Kagome VMC sampler (declared in include/kagome_vmc.h).
+//
+// The handle owns every device buffer of a walker batch.  Host entry points
+// validate their arguments, enqueue kernels on the handle's stream and
+// synchronise only where host data is returned or a refresh decision has to
+// be read back.
+#include "../../include/kagome_vmc.h"
+#include "kvmc_internal.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace kvmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+// Random starts whose pivots span more than this ratio are redrawn: such a
+// tildeU is numerically singular even when no pivot is exactly zero.
+constexpr double kStartPivotTol = 1e-10;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(KVMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(KVMC_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                        " bytes failed: " + hipGetErrorString(e));
+    }
+    return KVMC_OK;
+}
+
+// Seeded uniform Sz=0 configuration (Fisher-Yates over Philox words keyed
+// (seed, walker, attempt)).
+void random_config(uint64_t seed, uint32_t walker, uint32_t attempt, int ns, int nup, int ndn,
+                   int* kup, int* kdn) {
+    std::vector<int> perm(ns);
+    for (int i = 0; i < ns; ++i) perm[i] = i;
+    uint32_t words[4];
+    int widx = 4;
+    uint32_t call = 0;
+    for (int i = ns - 1; i >= 1; --i) {
+        if (widx == 4) {
+            philox4x32_10(call++, attempt, walker, 1u, (uint32_t)seed, (uint32_t)(seed >> 32), words);
+            widx = 0;
+        }
+        int j = (int)(((uint64_t)words[widx++] * (uint64_t)(i + 1)) >> 32);
+        std::swap(perm[i], perm[j]);
+    }
+    std::fill(kup, kup + ns, 0);
+    std::fill(kdn, kdn + ns, 0);
+    for (int a = 0; a < nup; ++a) kup[perm[a]] = a + 1;
+    for (int b = 0; b < ndn; ++b) kdn[perm[nup + b]] = b + 1;
+}
+
+__global__ void k_set_stop(const long long* sweeps, long long* stop, long long* base,
+                           long long nsweeps, int nw) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) {
+        base[w] = sweeps[w];
+        stop[w] = sweeps[w] + nsweeps;
+    }
+}
+
+__global__ void k_fill_ll(long long* p, long long v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i(int* p, int v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+struct kvmc_handle {
+    int ns = 0, nup = 0, ndn = 0, nb = 0, nmax = 0, nsp = 0;
+    int nw = 0;
+    int prec = 0, kernel = 0, policy = 0, refresh_k = 0, delay = 1, threads = 0;
+    uint64_t seed = 0;
+    size_t esize = 8;
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    int2* bonds = nullptr;
+    int* nbr_off = nullptr;
+    int* nbr = nullptr;
+    double* Uup = nullptr;
+    double* Udn = nullptr;
+    int* kup = nullptr;
+    int* kdn = nullptr;
+    void* W = nullptr;
+    void* A = nullptr;
+    void* Bm = nullptr;
+    int* npend = nullptr;
+    int* zmu = nullptr;
+    int* acc_since = nullptr;
+    int* status = nullptr;
+    long long* sweeps = nullptr;
+    long long* stop = nullptr;
+    long long* tbase = nullptr;
+    int* rlist = nullptr;
+    int* rcount = nullptr;
+    int* failed = nullptr;
+    int* all_walkers = nullptr;
+    unsigned long long* counters = nullptr;
+    double* logdet = nullptr;
+    double* eloc = nullptr;
+    double* drift = nullptr;
+    double* ws = nullptr;
+    int ws_jobs = 0;
+    int nparams = 0;
+    double* Qup = nullptr;
+    double* Qdn = nullptr;
+    double* O = nullptr;
+    int64_t failed_walker = -1;
+    bool profiling = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    kvmc_profile prof{};
+};
+
+namespace {
+
+void free_all(kvmc_handle* h) {
+    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->kup, h->kdn, h->W, h->A,
+                    h->Bm, h->npend, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
+                    h->tbase, h->rlist, h->rcount, h->failed, h->all_walkers, h->counters,
+                    h->logdet, h->eloc, h->drift, h->ws, h->Qup, h->Qdn, h->O};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// Event pair around a launch sequence on the handle's stream; the elapsed
+// time is read after the stream has been synchronised by the caller.
+void prof_begin(kvmc_handle* h) {
+    if (h->profiling) (void)hipEventRecord(h->ev0, h->stream);
+}
+void prof_end(kvmc_handle* h) {
+    if (h->profiling) (void)hipEventRecord(h->ev1, h->stream);
+}
+float prof_elapsed(kvmc_handle* h) {
+    float ms = 0.f;
+    if (h->profiling && hipEventSynchronize(h->ev1) == hipSuccess)
+        (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    return ms;
+}
+
+SweepArgs sweep_args(kvmc_handle* h) {
+    SweepArgs a{};
+    a.ns = h->ns; a.nup = h->nup; a.ndn = h->ndn; a.nb = h->nb;
+    a.nocc = std::min(h->nup, h->ndn);
+    a.bonds = h->bonds; a.nbr_off = h->nbr_off; a.nbr = h->nbr;
+    a.kup = h->kup; a.kdn = h->kdn; a.W = h->W; a.A = h->A; a.Bm = h->Bm;
+    a.npend = h->npend; a.delay = h->delay; a.nmax = h->nmax;
+    a.sweeps = h->sweeps; a.stop = h->stop; a.tape_base = h->tbase;
+    a.zmu = h->zmu; a.acc_since = h->acc_since;
+    a.refresh_list = h->rlist; a.refresh_count = h->rcount;
+    a.counters = h->counters; a.seed = h->seed;
+    a.kernel_mode = h->kernel; a.policy = h->policy; a.refresh_k = h->refresh_k;
+    return a;
+}
+
+// Refreshes the walkers listed in d_list[0..count) (device array).  When
+// write_w is 0 only the drift against the current W is measured.
+int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
+                 double pivot_tol = 0.0) {
+    if (count <= 0) return KVMC_OK;
+    static const int kNone = INT_MAX;
+    HIP_TRY(hipMemcpyAsync(h->failed, &kNone, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    RefreshArgs r{};
+    r.ns = h->ns; r.nup = h->nup; r.ndn = h->ndn; r.Uup = h->Uup; r.Udn = h->Udn;
+    r.kup = h->kup; r.kdn = h->kdn; r.W = h->W; r.A = h->A; r.Bm = h->Bm;
+    r.npend = h->npend; r.acc_since = h->acc_since; r.delay = h->delay; r.nmax = h->nmax;
+    r.ws = h->ws; r.logdet = h->logdet; r.status = h->status; r.failed = h->failed;
+    r.write_w = write_w; r.drift = h->drift; r.pivot_tol = pivot_tol;
+    const int batch_walkers = std::max(1, h->ws_jobs / 2);
+    prof_begin(h);
+    for (int b0 = 0; b0 < count; b0 += batch_walkers) {
+        const int nwb = std::min(batch_walkers, count - b0);
+        r.jobs = d_list + b0;
+        r.njobs = 2 * nwb;
+        HIP_TRY(launch_refresh(h->prec, r, h->stream));
+        h->prof.refresh_launches += h->profiling;
+    }
+    prof_end(h);
+    int failed = INT_MAX;
+    HIP_TRY(hipMemcpyAsync(&failed, h->failed, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->prof.refresh_ms += prof_elapsed(h);
+    if (failed != INT_MAX) {
+        h->failed_walker = failed;
+        return fail(KVMC_ERR_SINGULAR, "tildeU is singular for walker " + std::to_string(failed) +
+                                           " (lu factorization failed)");
+    }
+    return KVMC_OK;
+}
+
+int refresh_all(kvmc_handle* h, int write_w) {
+    HIP_TRY(hipMemsetAsync(h->status, 0, sizeof(int) * h->nw, h->stream));
+    return refresh_list(h, h->all_walkers, h->nw, write_w);
+}
+
+int validate_configs(const kvmc_handle* h, const int32_t* ku, const int32_t* kd, int64_t w) {
+    std::vector<char> seen_u(h->nup + 1, 0), seen_d(h->ndn + 1, 0);
+    for (int s = 0; s < h->ns; ++s) {
+        int lu = ku[s], ld = kd[s];
+        if ((lu != 0) == (ld != 0))
+            return fail(KVMC_ERR_INVALID, "walker " + std::to_string(w) + ": site " +
+                                              std::to_string(s + 1) +
+                                              ((lu != 0) ? " is doubly occupied" : " is unoccupied"));
+        if (lu < 0 || lu > h->nup || ld < 0 || ld > h->ndn)
+            return fail(KVMC_ERR_INVALID, "walker " + std::to_string(w) + ": label out of bounds");
+        if (lu) { if (seen_u[lu]) return fail(KVMC_ERR_INVALID, "walker " + std::to_string(w) + ": repeated up label"); seen_u[lu] = 1; }
+        if (ld) { if (seen_d[ld]) return fail(KVMC_ERR_INVALID, "walker " + std::to_string(w) + ": repeated down label"); seen_d[ld] = 1; }
+    }
+    for (int l = 1; l <= h->nup; ++l)
+        if (!seen_u[l]) return fail(KVMC_ERR_INVALID, "walker " + std::to_string(w) + ": kappa_up is not valid");
+    for (int l = 1; l <= h->ndn; ++l)
+        if (!seen_d[l]) return fail(KVMC_ERR_INVALID, "walker " + std::to_string(w) + ": kappa_down is not valid");
+    return KVMC_OK;
+}
+
+int upload_configs(kvmc_handle* h, const int32_t* ku, const int32_t* kd) {
+    const size_t n = (size_t)h->nw * h->ns;
+    HIP_TRY(hipMemcpyAsync(h->kup, ku, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->kdn, kd, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemsetAsync(h->npend, 0, sizeof(int) * h->nw, h->stream));
+    HIP_TRY(hipMemsetAsync(h->acc_since, 0, sizeof(int) * h->nw, h->stream));
+    return KVMC_OK;
+}
+
+int sweep_threads(const kvmc_handle* h) {
+    const int V = h->prec == 0 ? 2 : 4;
+    const int nvec = (h->ns % V == 0) ? h->ns / V : h->ns;
+    int t = h->threads > 0 ? h->threads : 256;
+    t = std::max(t, (nvec + 63) / 64 * 64);
+    return std::min(t, 512);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kvmc_last_error(void) { return g_err.c_str(); }
+
+const char* kvmc_version(void) { return "kagome_vmc 0.1 gfx950"; }
+
+int64_t kvmc_failed_walker(kvmc_handle* h) { return h ? h->failed_walker : -1; }
+
+int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
+    g_err.clear();
+    if (!d || !out) return fail(KVMC_ERR_INVALID, "null descriptor or output");
+    *out = nullptr;
+    if (d->ns <= 0 || d->n_up <= 0 || d->n_down <= 0 || d->n_up + d->n_down != d->ns)
+        return fail(KVMC_ERR_INVALID, "need n_up + n_down == ns (one spinon per site)");
+    if (d->nb <= 0 || !d->bonds || !d->u_up || !d->u_down)
+        return fail(KVMC_ERR_INVALID, "bonds and orbitals are required");
+    if (d->nwalkers <= 0 || d->nwalkers > INT_MAX / 2)
+        return fail(KVMC_ERR_INVALID, "nwalkers out of range");
+    if (d->precision != KVMC_PREC_FP64 && d->precision != KVMC_PREC_FP32)
+        return fail(KVMC_ERR_INVALID, "unknown precision");
+    if (d->kernel != KVMC_KERNEL_REFERENCE && d->kernel != KVMC_KERNEL_UNBIASED)
+        return fail(KVMC_ERR_INVALID, "unknown transition kernel");
+    if (d->refresh_policy != KVMC_REFRESH_REFERENCE && d->refresh_policy != KVMC_REFRESH_EVERY_K)
+        return fail(KVMC_ERR_INVALID, "unknown refresh policy");
+    if (d->refresh_policy == KVMC_REFRESH_EVERY_K && d->refresh_k <= 0)
+        return fail(KVMC_ERR_INVALID, "refresh_k must be positive");
+    const int delay = d->delay <= 1 ? 1 : d->delay;
+    if (delay > kMaxDelay)
+        return fail(KVMC_ERR_INVALID, "delay must be <= " + std::to_string(kMaxDelay));
+    if (d->ns > 4096) return fail(KVMC_ERR_INVALID, "ns > 4096 is not supported");
+    for (int b = 0; b < d->nb; ++b) {
+        int i = d->bonds[2 * b], j = d->bonds[2 * b + 1];
+        if (i < 1 || j < 1 || i > d->ns || j > d->ns || i > j)
+            return fail(KVMC_ERR_INVALID, "bond " + std::to_string(b + 1) + " out of range");
+    }
+    kvmc_handle* h = new kvmc_handle();
+    h->ns = d->ns; h->nup = d->n_up; h->ndn = d->n_down; h->nb = d->nb;
+    h->nmax = std::max(h->nup, h->ndn);
+    h->nsp = (h->ns + 3) / 4 * 4;
+    h->nw = (int)d->nwalkers;
+    h->prec = d->precision; h->kernel = d->kernel; h->policy = d->refresh_policy;
+    h->refresh_k = d->refresh_k; h->delay = delay; h->threads = d->threads; h->seed = d->seed;
+    h->esize = h->prec == KVMC_PREC_FP64 ? sizeof(double) : sizeof(float);
+    (void)hipGetDevice(&h->device);
+    int rc = KVMC_OK;
+#define ALLOC(p, n) do { if ((rc = dalloc(&(p), (n))) != KVMC_OK) { free_all(h); delete h; return rc; } } while (0)
+    hipError_t e = hipStreamCreate(&h->stream);
+    if (e != hipSuccess) { delete h; return fail(KVMC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    const size_t nw = (size_t)h->nw, ns = (size_t)h->ns;
+    ALLOC(h->bonds, (size_t)h->nb);
+    ALLOC(h->nbr_off, ns + 1);
+    ALLOC(h->nbr, (size_t)2 * h->nb);
+    ALLOC(h->Uup, ns * h->nup);
+    ALLOC(h->Udn, ns * h->ndn);
+    ALLOC(h->kup, nw * ns);
+    ALLOC(h->kdn, nw * ns);
+    {
+        unsigned char* p = nullptr;
+        ALLOC(p, nw * ns * (h->nup + h->ndn) * h->esize); h->W = p;
+        ALLOC(p, nw * 2 * delay * h->nsp * h->esize); h->A = p;
+        ALLOC(p, nw * 2 * delay * h->nmax * h->esize); h->Bm = p;
+    }
+    ALLOC(h->npend, nw); ALLOC(h->zmu, nw); ALLOC(h->acc_since, nw); ALLOC(h->status, nw);
+    ALLOC(h->sweeps, nw); ALLOC(h->stop, nw); ALLOC(h->tbase, nw);
+    ALLOC(h->rlist, nw); ALLOC(h->rcount, 1); ALLOC(h->failed, 1); ALLOC(h->all_walkers, nw);
+    ALLOC(h->counters, (size_t)CNT_SLOTS);
+    ALLOC(h->logdet, 2 * nw); ALLOC(h->eloc, nw); ALLOC(h->drift, 2);
+    // refresh workspace: up to ~1 GiB of FP64 work matrices per batch
+    {
+        const size_t per_job = ns * h->nmax * sizeof(double);
+        size_t jobs = std::max<size_t>(2, ((size_t)1 << 30) / per_job);
+        jobs = std::min(jobs, 2 * nw);
+        h->ws_jobs = (int)(jobs & ~(size_t)1);
+        if (h->ws_jobs < 2) h->ws_jobs = 2;
+        ALLOC(h->ws, (size_t)h->ws_jobs * ns * h->nmax);
+    }
+#undef ALLOC
+    // bonds (0-based) and site -> bond adjacency
+    std::vector<int2> hb(h->nb);
+    std::vector<std::vector<int>> adj(ns);
+    for (int b = 0; b < h->nb; ++b) {
+        hb[b] = make_int2(d->bonds[2 * b] - 1, d->bonds[2 * b + 1] - 1);
+        adj[hb[b].x].push_back(b);
+        if (hb[b].y != hb[b].x) adj[hb[b].y].push_back(b);
+    }
+    std::vector<int> off(ns + 1, 0), lst;
+    for (size_t s = 0; s < ns; ++s) {
+        off[s + 1] = off[s] + (int)adj[s].size();
+        lst.insert(lst.end(), adj[s].begin(), adj[s].end());
+    }
+    std::vector<int> allw(nw);
+    for (size_t w = 0; w < nw; ++w) allw[w] = (int)w;
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream);
+    };
+    if (up(h->bonds, hb.data(), hb.size() * sizeof(int2)) != hipSuccess ||
+        up(h->nbr_off, off.data(), off.size() * sizeof(int)) != hipSuccess ||
+        (lst.size() && up(h->nbr, lst.data(), lst.size() * sizeof(int)) != hipSuccess) ||
+        up(h->Uup, d->u_up, ns * h->nup * sizeof(double)) != hipSuccess ||
+        up(h->Udn, d->u_down, ns * h->ndn * sizeof(double)) != hipSuccess ||
+        up(h->all_walkers, allw.data(), nw * sizeof(int)) != hipSuccess ||
+        hipMemsetAsync(h->sweeps, 0, nw * sizeof(long long), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->npend, 0, nw * sizeof(int), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->acc_since, 0, nw * sizeof(int), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->logdet, 0, 2 * nw * sizeof(double), h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        free_all(h);
+        delete h;
+        return fail(KVMC_ERR_HIP, "uploading lattice data failed");
+    }
+    *out = h;
+    return KVMC_OK;
+}
+
+int kvmc_destroy(kvmc_handle* h) {
+    if (!h) return KVMC_OK;
+    (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+    return KVMC_OK;
+}
+
+int kvmc_set_configs(kvmc_handle* h, const int32_t* ku, const int32_t* kd) {
+    g_err.clear();
+    if (!h || !ku || !kd) return fail(KVMC_ERR_INVALID, "null argument");
+    for (int w = 0; w < h->nw; ++w) {
+        int rc = validate_configs(h, ku + (size_t)w * h->ns, kd + (size_t)w * h->ns, w);
+        if (rc) return rc;
+    }
+    int rc = upload_configs(h, ku, kd);
+    if (rc) return rc;
+    h->failed_walker = -1;
+    rc = refresh_all(h, 1);
+    h->ready = rc == KVMC_OK;
+    return rc;
+}
+
+int kvmc_get_configs(kvmc_handle* h, int32_t* ku, int32_t* kd) {
+    g_err.clear();
+    if (!h || !ku || !kd) return fail(KVMC_ERR_INVALID, "null argument");
+    const size_t n = (size_t)h->nw * h->ns;
+    HIP_TRY(hipMemcpyAsync(ku, h->kup, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(kd, h->kdn, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return KVMC_OK;
+}
+
+int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_t* redraws) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (max_attempts < 1) max_attempts = 1;
+    const size_t ns = h->ns;
+    std::vector<int> ku((size_t)h->nw * ns), kd((size_t)h->nw * ns);
+    for (int w = 0; w < h->nw; ++w)
+        random_config(seed, (uint32_t)w, 0u, h->ns, h->nup, h->ndn, &ku[w * ns], &kd[w * ns]);
+    int rc = upload_configs(h, ku.data(), kd.data());
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(h->status, 0, sizeof(int) * h->nw, h->stream));
+    rc = refresh_list(h, h->all_walkers, h->nw, 1, kStartPivotTol);
+    if (rc != KVMC_OK && rc != KVMC_ERR_SINGULAR) return rc;
+    int64_t nredraw = 0;
+    std::vector<int> status(h->nw);
+    for (int attempt = 1; rc == KVMC_ERR_SINGULAR; ++attempt) {
+        HIP_TRY(hipMemcpyAsync(status.data(), h->status, sizeof(int) * h->nw, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        std::vector<int> bad;
+        for (int w = 0; w < h->nw; ++w)
+            if (status[w]) bad.push_back(w);
+        if (bad.empty()) { rc = KVMC_OK; break; }
+        if (attempt >= max_attempts)
+            return fail(KVMC_ERR_SINGULAR, std::to_string(bad.size()) +
+                                               " walkers still singular after " +
+                                               std::to_string(max_attempts) + " draws");
+        for (int w : bad) {
+            random_config(seed, (uint32_t)w, (uint32_t)attempt, h->ns, h->nup, h->ndn, &ku[w * ns], &kd[w * ns]);
+            HIP_TRY(hipMemcpyAsync(h->kup + w * ns, &ku[w * ns], ns * sizeof(int), hipMemcpyHostToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(h->kdn + w * ns, &kd[w * ns], ns * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        }
+        nredraw += (int64_t)bad.size();
+        int* dbad = nullptr;
+        if ((rc = dalloc(&dbad, bad.size())) != KVMC_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(dbad, bad.data(), bad.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        for (int w : bad) status[w] = 0;
+        HIP_TRY(hipMemcpyAsync(h->status, status.data(), sizeof(int) * h->nw, hipMemcpyHostToDevice, h->stream));
+        rc = refresh_list(h, dbad, (int)bad.size(), 1, kStartPivotTol);
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        (void)hipFree(dbad);
+        if (rc != KVMC_OK && rc != KVMC_ERR_SINGULAR) return rc;
+    }
+    if (redraws) *redraws = nredraw;
+    h->failed_walker = -1;
+    h->ready = true;
+    g_err.clear();
+    return KVMC_OK;
+}
+
+int kvmc_refresh(kvmc_handle* h) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
+    return refresh_all(h, 1);
+}
+
+int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
+               int8_t* outcomes, kvmc_counters* counters) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
+    if (nsweeps < 0) return fail(KVMC_ERR_INVALID, "nsweeps < 0");
+    if ((tape_u == nullptr) != (tape_bond == nullptr))
+        return fail(KVMC_ERR_INVALID, "tape_u and tape_bond go together");
+    if (nsweeps == 0) { if (counters) *counters = kvmc_counters{}; return KVMC_OK; }
+    const size_t tape_n = (size_t)h->nw * (size_t)nsweeps;
+    if (tape_u)
+        for (size_t q = 0; q < tape_n; ++q)
+            if (tape_bond[q] < 0 || tape_bond[q] >= h->nb || !(tape_u[q] >= 0.0 && tape_u[q] < 1.0))
+                return fail(KVMC_ERR_INVALID, "tape entry " + std::to_string(q) + " out of range");
+    double* d_tu = nullptr;
+    int* d_tb = nullptr;
+    signed char* d_out = nullptr;
+    int rc = KVMC_OK;
+    auto cleanup = [&]() {
+        if (d_tu) (void)hipFree(d_tu);
+        if (d_tb) (void)hipFree(d_tb);
+        if (d_out) (void)hipFree(d_out);
+    };
+    if (tape_u) {
+        if ((rc = dalloc(&d_tu, tape_n)) || (rc = dalloc(&d_tb, tape_n))) { cleanup(); return rc; }
+        if (hipMemcpyAsync(d_tu, tape_u, tape_n * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(d_tb, tape_bond, tape_n * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            cleanup();
+            return fail(KVMC_ERR_HIP, "tape upload failed");
+        }
+    }
+    if (outcomes && (rc = dalloc(&d_out, tape_n))) { cleanup(); return rc; }
+    SweepArgs a = sweep_args(h);
+    a.tape_u = d_tu; a.tape_bond = d_tb; a.tape_len = nsweeps; a.outcomes = d_out;
+    a.walker0 = 0;
+    const int threads = sweep_threads(h);
+    const size_t lds = sweep_lds_bytes(h->prec, h->ns, h->nup, h->ndn, h->delay);
+    if (lds > 160 * 1024) { cleanup(); return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay"); }
+    int64_t launches = 0, refreshes = 0;
+    hipStream_t s = h->stream;
+    {
+        hipLaunchKernelGGL(k_set_stop, dim3((h->nw + 255) / 256), dim3(256), 0, s,
+                           h->sweeps, h->stop, h->tbase, (long long)nsweeps, h->nw);
+        if (hipMemsetAsync(h->counters, 0, sizeof(unsigned long long) * CNT_SLOTS, s) != hipSuccess) {
+            cleanup(); return fail(KVMC_ERR_HIP, "counter reset failed");
+        }
+    }
+    for (;;) {
+        int count = 0;
+        if (hipMemsetAsync(h->rcount, 0, sizeof(int), s) != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, "hipMemsetAsync failed"); }
+        prof_begin(h);
+        hipError_t e = launch_sweep(h->prec, a, h->nw, threads, lds, s);
+        if (e != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e)); }
+        prof_end(h);
+        ++launches;
+        if (hipMemcpyAsync(&count, h->rcount, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            cleanup();
+            return fail(KVMC_ERR_HIP, std::string("sweep kernel: ") + hipGetErrorString(hipGetLastError()));
+        }
+        h->prof.sweep_ms += prof_elapsed(h);
+        h->prof.sweep_launches += h->profiling;
+        if (count == 0) break;
+        refreshes += count;
+        rc = refresh_list(h, h->rlist, count, 1);
+        if (rc) { cleanup(); return rc; }
+    }
+    if (outcomes &&
+        hipMemcpyAsync(outcomes, d_out, tape_n, hipMemcpyDeviceToHost, s) != hipSuccess) {
+        cleanup(); return fail(KVMC_ERR_HIP, "outcome copy failed");
+    }
+    unsigned long long cnt[CNT_SLOTS] = {};
+    if (hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        cleanup(); return fail(KVMC_ERR_HIP, "counter copy failed");
+    }
+    cleanup();
+    if (counters) {
+        counters->sweeps = (int64_t)cnt[CNT_SWEEPS];
+        counters->moves = (int64_t)cnt[CNT_MOVES];
+        counters->accepted = (int64_t)cnt[CNT_ACCEPTED];
+        counters->refreshes = refreshes;
+        counters->flushes = (int64_t)cnt[CNT_FLUSHES];
+        counters->launches = launches;
+    }
+    return KVMC_OK;
+}
+
+static ElocArgs eloc_args(kvmc_handle* h) {
+    ElocArgs e{};
+    e.ns = h->ns; e.nup = h->nup; e.ndn = h->ndn; e.nb = h->nb;
+    e.bonds = h->bonds; e.kup = h->kup; e.kdn = h->kdn; e.W = h->W; e.A = h->A; e.Bm = h->Bm;
+    e.npend = h->npend; e.delay = h->delay; e.nmax = h->nmax; e.nw = h->nw;
+    e.eloc = h->eloc; e.counters = h->counters;
+    e.nparams = h->nparams; e.Qup = h->Qup; e.Qdn = h->Qdn; e.O = h->O;
+    return e;
+}
+
+int kvmc_measure(kvmc_handle* h, double* eloc, double* sum, int64_t* n_ap) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemsetAsync(h->counters + CNT_NAP, 0, sizeof(unsigned long long), s));
+    prof_begin(h);
+    HIP_TRY(launch_eloc(h->prec, eloc_args(h), s));
+    prof_end(h);
+    std::vector<double> tmp;
+    double* dst = eloc;
+    if (!dst && sum) { tmp.resize(h->nw); dst = tmp.data(); }
+    if (dst) HIP_TRY(hipMemcpyAsync(dst, h->eloc, sizeof(double) * h->nw, hipMemcpyDeviceToHost, s));
+    unsigned long long nap = 0;
+    HIP_TRY(hipMemcpyAsync(&nap, h->counters + CNT_NAP, sizeof(nap), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    h->prof.measure_ms += prof_elapsed(h);
+    h->prof.measure_launches += h->profiling;
+    if (sum) {
+        double acc = 0.0;
+        for (int w = 0; w < h->nw; ++w) acc += dst[w];
+        *sum = acc;
+    }
+    if (n_ap) *n_ap = (int64_t)nap;
+    return KVMC_OK;
+}
+
+int kvmc_set_orbital_derivs(kvmc_handle* h, int32_t nparams, const double* du_up,
+                            const double* du_dn) {
+    g_err.clear();
+    if (!h || nparams < 0 || (nparams > 0 && (!du_up || !du_dn)))
+        return fail(KVMC_ERR_INVALID, "bad orbital-derivative arguments");
+    if (h->Qup) { (void)hipFree(h->Qup); h->Qup = nullptr; }
+    if (h->Qdn) { (void)hipFree(h->Qdn); h->Qdn = nullptr; }
+    if (h->O) { (void)hipFree(h->O); h->O = nullptr; }
+    h->nparams = 0;
+    if (nparams == 0) return KVMC_OK;
+    const size_t ns = h->ns;
+    int rc;
+    double *dU = nullptr;
+    if ((rc = dalloc(&h->Qup, (size_t)nparams * ns * ns)) || (rc = dalloc(&h->Qdn, (size_t)nparams * ns * ns)) ||
+        (rc = dalloc(&h->O, (size_t)nparams * h->nw)) || (rc = dalloc(&dU, (size_t)nparams * ns * h->nmax)))
+        return rc;
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(dU, du_up, (size_t)nparams * ns * h->nup * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_q_build(dU, h->Uup, h->ns, h->nup, nparams, h->Qup, s));
+    HIP_TRY(hipMemcpyAsync(dU, du_dn, (size_t)nparams * ns * h->ndn * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_q_build(dU, h->Udn, h->ns, h->ndn, nparams, h->Qdn, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    (void)hipFree(dU);
+    h->nparams = nparams;
+    return KVMC_OK;
+}
+
+int kvmc_measure_grad(kvmc_handle* h, double* eloc, double* o) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
+    if (h->nparams <= 0) return fail(KVMC_ERR_INVALID, "orbital derivatives not set");
+    hipStream_t s = h->stream;
+    SweepArgs a = sweep_args(h);
+    HIP_TRY(launch_flush(h->prec, a, h->nw, s));
+    HIP_TRY(launch_eloc(h->prec, eloc_args(h), s));
+    HIP_TRY(launch_grad(h->prec, eloc_args(h), s));
+    if (eloc) HIP_TRY(hipMemcpyAsync(eloc, h->eloc, sizeof(double) * h->nw, hipMemcpyDeviceToHost, s));
+    if (o) HIP_TRY(hipMemcpyAsync(o, h->O, sizeof(double) * h->nw * h->nparams, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return KVMC_OK;
+}
+
+int kvmc_get_logdet(kvmc_handle* h, double* logdet) {
+    g_err.clear();
+    if (!h || !logdet) return fail(KVMC_ERR_INVALID, "null argument");
+    std::vector<double> two((size_t)2 * h->nw);
+    HIP_TRY(hipMemcpyAsync(two.data(), h->logdet, two.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int w = 0; w < h->nw; ++w) logdet[w] = two[2 * w] + two[2 * w + 1];
+    return KVMC_OK;
+}
+
+int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out) {
+    g_err.clear();
+    if (!h || !out || walker < 0 || walker >= h->nw || spin < 0 || spin > 1)
+        return fail(KVMC_ERR_INVALID, "bad walker/spin");
+    hipStream_t s = h->stream;
+    SweepArgs a = sweep_args(h);
+    a.walker0 = (int)walker;
+    HIP_TRY(launch_flush(h->prec, a, 1, s));
+    const size_t ns = h->ns, n = spin ? h->ndn : h->nup;
+    const size_t off = (size_t)walker * ns * (h->nup + h->ndn) + (spin ? ns * h->nup : 0);
+    if (h->prec == KVMC_PREC_FP64) {
+        HIP_TRY(hipMemcpyAsync(out, (double*)h->W + off, ns * n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        std::vector<float> tmp(ns * n);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)h->W + off, ns * n * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (size_t q = 0; q < ns * n; ++q) out[q] = tmp[q];
+    }
+    return KVMC_OK;
+}
+
+int kvmc_set_sweep_count(kvmc_handle* h, int64_t sweeps) {
+    g_err.clear();
+    if (!h || sweeps < 0) return fail(KVMC_ERR_INVALID, "bad argument");
+    hipLaunchKernelGGL(k_fill_ll, dim3((h->nw + 255) / 256), dim3(256), 0, h->stream, h->sweeps,
+                       (long long)sweeps, h->nw);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return KVMC_OK;
+}
+
+int kvmc_get_sweep_count(kvmc_handle* h, int64_t* sweeps) {
+    g_err.clear();
+    if (!h || !sweeps) return fail(KVMC_ERR_INVALID, "bad argument");
+    HIP_TRY(hipMemcpyAsync(sweeps, h->sweeps, sizeof(long long) * h->nw, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return KVMC_OK;
+}
+
+int kvmc_profile_enable(kvmc_handle* h, int32_t enable) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (enable && !h->ev0) {
+        HIP_TRY(hipEventCreate(&h->ev0));
+        HIP_TRY(hipEventCreate(&h->ev1));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->profiling = enable != 0;
+    h->prof = kvmc_profile{};
+    return KVMC_OK;
+}
+
+int kvmc_get_profile(kvmc_handle* h, kvmc_profile* out) {
+    g_err.clear();
+    if (!h || !out) return fail(KVMC_ERR_INVALID, "null argument");
+    *out = h->prof;
+    return KVMC_OK;
+}
+
+int kvmc_w_drift(kvmc_handle* h, double* max_rel) {
+    g_err.clear();
+    if (!h || !max_rel) return fail(KVMC_ERR_INVALID, "null argument");
+    if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
+    HIP_TRY(hipMemsetAsync(h->drift, 0, 2 * sizeof(double), h->stream));
+    int rc = refresh_all(h, 0);
+    if (rc) return rc;
+    double dr[2] = {0, 0};
+    HIP_TRY(hipMemcpy(dr, h->drift, sizeof(dr), hipMemcpyDeviceToHost));
+    *max_rel = dr[1] > 0 ? dr[0] / dr[1] : 0.0;
+    return KVMC_OK;
+}
+
+}  // extern "C"

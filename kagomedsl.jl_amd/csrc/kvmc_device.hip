@@ -1,0 +1,715 @@
+// Device kernels of the Kagome VMC sampler, written for gfx950 (CDNA4).
+//
+//   k_sweep    Metropolis spin-exchange chain per walker (one workgroup per
+//              walker, many sweeps per launch) fused with the Sherman-Morrison
+//              W update.  The update is kept as `delay` pending rank-1 terms
+//              (columns A_q, scaled rows B_q in LDS) and applied to W in HBM
+//              by one streaming rank-`delay` pass; delay = 1 is the
+//              reference's immediate geru (src/MonteCarlo.jl:279-292).
+//   k_refresh  FP64 pivoted LU of tildeU and W = U tildeU^-1
+//              (src/MonteCarlo.jl:55-115), one workgroup per (walker, spin).
+//   k_eloc     Heisenberg J1 local energy (src/Hamiltonian.jl:762-778), one
+//              wavefront per walker, FP64 accumulation.
+//   k_grad     log-derivatives O_k = d log det tildeU / d theta_k.
+//   k_flush    applies pending delayed updates to W.
+//
+// Layout in HBM (per walker, walker-major): kappa_up[ns], kappa_dn[ns]
+// (int32, 1-based labels, 0 = empty); W_up (ns x nup) then W_dn (ns x ndn),
+// column-major, so column l (a label) is contiguous.
+#include "kvmc_internal.hpp"
+
+#include <float.h>
+#include <math.h>
+
+namespace kvmc {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <typename T, int V> struct VecT;
+template <> struct VecT<float, 4> { using type = f32x4; };
+template <> struct VecT<double, 2> { using type = f64x2; };
+template <> struct VecT<float, 1> { using type = float; };
+template <> struct VecT<double, 1> { using type = double; };
+
+template <typename T> constexpr int native_vec() { return 16 / (int)sizeof(T); }
+
+__device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// Streaming rank-c update of one spin's W:  W[:, j] += sum_{q<c} A_q * B_q[j].
+// Thread t owns the row vector iv = t % nvec of every column j == lane (mod
+// lanes); its A_q slices stay in registers for the whole pass, B_q[j] is an
+// LDS broadcast.  Requires blockDim.x >= ns / V (checked on the host).
+// ---------------------------------------------------------------------------
+template <typename T, int V, int DMAX>
+__device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __restrict__ sA,
+                                              const T* __restrict__ sB, int ns, int nsp,
+                                              int nmax, int ncols, int c) {
+    using VT = typename VecT<T, V>::type;
+    const int nvec = ns / V;
+    const int lanes = (int)blockDim.x / nvec;
+    const int t = (int)threadIdx.x;
+    const int iv = t % nvec, lane = t / nvec;
+    if (lane >= lanes) return;
+    VT a[DMAX];
+#pragma unroll
+    for (int q = 0; q < DMAX; ++q)
+        if (q < c) a[q] = *reinterpret_cast<const VT*>(sA + (size_t)q * nsp + (size_t)iv * V);
+    T* base = W0 + (size_t)iv * V;
+    int j = lane;
+    for (; j + lanes < ncols; j += 2 * lanes) {
+        VT w0 = *reinterpret_cast<VT*>(base + (size_t)j * ns);
+        VT w1 = *reinterpret_cast<VT*>(base + (size_t)(j + lanes) * ns);
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) {
+            if (q < c) {
+                w0 += a[q] * sB[q * nmax + j];
+                w1 += a[q] * sB[q * nmax + j + lanes];
+            }
+        }
+        *reinterpret_cast<VT*>(base + (size_t)j * ns) = w0;
+        *reinterpret_cast<VT*>(base + (size_t)(j + lanes) * ns) = w1;
+    }
+    if (j < ncols) {
+        VT w0 = *reinterpret_cast<VT*>(base + (size_t)j * ns);
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q)
+            if (q < c) w0 += a[q] * sB[q * nmax + j];
+        *reinterpret_cast<VT*>(base + (size_t)j * ns) = w0;
+    }
+}
+
+// Current W element: W0[K, l] + sum_{q<c} A_q[K] B_q[l].
+template <typename T>
+__device__ __forceinline__ T w_cur(const T* __restrict__ W0, const T* sA, const T* sB, int ns,
+                                   int nsp, int nmax, int K, int l, int c) {
+    T v = W0[(size_t)l * ns + K];
+    for (int q = 0; q < c; ++q) v += sA[q * nsp + K] * sB[q * nmax + l];
+    return v;
+}
+
+// Appends the rank-1 term of an accepted move on one spin (row K, label l,
+// both 0-based): A_c = W[:, l], B_c = alpha * (W[K, :] - e_l), alpha = -1/den
+// with den = W[K, l], exactly the reference's update_W! factors.
+template <typename T>
+__device__ __forceinline__ void append_term(const T* __restrict__ W0, T* sA, T* sB, int ns,
+                                            int nsp, int nmax, int ncols, int K, int l, int c,
+                                            T den) {
+    const T alpha = T(-1) / den;
+    T* An = sA + c * nsp;
+    T* Bn = sB + c * nmax;
+    for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
+        T v = W0[(size_t)l * ns + t];
+        for (int q = 0; q < c; ++q) v += sA[q * nsp + t] * sB[q * nmax + l];
+        An[t] = v;
+    }
+    for (int t = (int)threadIdx.x; t < ncols; t += (int)blockDim.x) {
+        T v = W0[(size_t)t * ns + K];
+        for (int q = 0; q < c; ++q) v += sA[q * nsp + K] * sB[q * nmax + t];
+        if (t == l) v -= T(1);
+        Bn[t] = alpha * v;
+    }
+}
+
+// Antiparallel bonds among those touching site i or site j (the only bonds
+// whose status an exchange of i and j can change).  Matches Z's test
+// (src/MonteCarlo.jl:460-474).
+__device__ __forceinline__ int count_touching(const int2* __restrict__ bonds,
+                                              const int* __restrict__ nbr_off,
+                                              const int* __restrict__ nbr, const int* s_kup,
+                                              const int* s_kdn, int i, int j) {
+    int cnt = 0;
+    for (int p = nbr_off[i]; p < nbr_off[i + 1]; ++p) {
+        int2 b = bonds[nbr[p]];
+        cnt += (s_kup[b.x] != 0 && s_kdn[b.y] != 0) || (s_kup[b.y] != 0 && s_kdn[b.x] != 0);
+    }
+    for (int p = nbr_off[j]; p < nbr_off[j + 1]; ++p) {
+        int2 b = bonds[nbr[p]];
+        if (b.x == i || b.y == i) continue;
+        cnt += (s_kup[b.x] != 0 && s_kdn[b.y] != 0) || (s_kup[b.y] != 0 && s_kdn[b.x] != 0);
+    }
+    return cnt;
+}
+
+__device__ inline double block_sum_double(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = ((int)blockDim.x + 63) / 64;
+    for (int q = 0; q < nw; ++q) s += red[q];
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Sweep kernel: Carlo.sweep! (src/MonteCarlo.jl:538-607) repeated until the
+// walker reaches its stop count or a refresh is due.  Every thread walks the
+// same (uniform) control flow; proposals are evaluated redundantly so no
+// barrier is needed on rejected sweeps.
+// ---------------------------------------------------------------------------
+template <typename T, int V, int DMAX>
+__global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int w = a.walker0 + (int)blockIdx.x;
+    const int ns = a.ns, nup = a.nup, ndn = a.ndn, nb = a.nb, nmax = a.nmax;
+    const int nsp = round_up(ns, 4);
+    const int delay = a.delay;
+    int* s_kup = reinterpret_cast<int*>(lds);
+    int* s_kdn = s_kup + nsp;
+    T* s_A = reinterpret_cast<T*>(s_kdn + nsp);           // [2][delay][nsp]
+    T* s_B = s_A + 2 * delay * nsp;                       // [2][delay][nmax]
+    double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
+
+    const size_t wstride = (size_t)ns * (nup + ndn);
+    T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
+    T* Wdn = Wup + (size_t)ns * nup;
+    T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsp;
+    T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
+    T* sAu = s_A;
+    T* sAd = s_A + delay * nsp;
+    T* sBu = s_B;
+    T* sBd = s_B + delay * nmax;
+
+    long long sweeps = a.sweeps[w];
+    const long long stop = a.stop[w];
+    if (sweeps >= stop) return;
+    int c = a.npend[w];
+    int acc_since = a.acc_since[w];
+
+    for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
+        s_kup[t] = a.kup[(size_t)w * ns + t];
+        s_kdn[t] = a.kdn[(size_t)w * ns + t];
+    }
+    if (c > 0) {
+        for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
+        for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
+    }
+    __syncthreads();
+    // Z_mu: full recount once per launch, then maintained incrementally.
+    int zmu;
+    {
+        double part = 0.0;
+        for (int b = (int)threadIdx.x; b < nb; b += (int)blockDim.x) {
+            int2 bd = a.bonds[b];
+            part += (s_kup[bd.x] != 0 && s_kdn[bd.y] != 0) ||
+                    (s_kup[bd.y] != 0 && s_kdn[bd.x] != 0);
+        }
+        zmu = (int)block_sum_double(part, s_red);
+    }
+    const double zmax = (double)nb;
+    const long long tbase = a.tape_base[w];
+    unsigned long long n_moves = 0, n_acc = 0, n_flush = 0, n_sweeps = 0;
+    bool pause = false;
+
+    while (sweeps < stop) {
+        double r;
+        int bidx;
+        const long long tidx = sweeps - tbase;
+        if (a.tape_u) {
+            r = a.tape_u[(size_t)w * a.tape_len + tidx];
+            bidx = a.tape_bond[(size_t)w * a.tape_len + tidx];
+        } else {
+            sweep_draws(a.seed, (uint32_t)w, sweeps, nb, &r, &bidx);
+        }
+        const double zr = (double)zmu / zmax;
+        int oc;
+        bool refresh_due = false;
+        if (a.kernel_mode == KERNEL_REFERENCE && r > zr) {
+            oc = OUT_PRE_REJECT;
+        } else {
+            const int2 bd = a.bonds[bidx];
+            const int i = bd.x, site = bd.y;
+            const int iu = s_kup[i], su = s_kup[site], id = s_kdn[i], sd = s_kdn[site];
+            const bool can1 = iu != 0 && sd != 0;
+            const bool can2 = su != 0 && id != 0;
+            if (!can1 && !can2) {
+                oc = OUT_NO_MOVE;
+            } else {
+                ++n_moves;
+                // flag = 1: up moves i -> site, down moves site -> i
+                const int lup = (can1 ? iu : su) - 1;
+                const int ldn = (can1 ? sd : id) - 1;
+                const int Kup = can1 ? site : i;
+                const int Kdn = can1 ? i : site;
+                const T wu = w_cur(Wup, sAu, sBu, ns, nsp, nmax, Kup, lup, c);
+                const T wd = w_cur(Wdn, sAd, sBd, ns, nsp, nmax, Kdn, ldn, c);
+                const T ratio = wu * wd;
+                const double acc_prob = (double)ratio * (double)ratio;
+                bool accept;
+                if (a.kernel_mode == KERNEL_REFERENCE)
+                    accept = (acc_prob >= 1.0 && r < zr) || (acc_prob < 1.0 && r < zr * acc_prob);
+                else
+                    accept = r < acc_prob;
+                oc = accept ? OUT_ACCEPT : OUT_REJECT;
+                if (accept) {
+                    ++n_acc;
+                    ++acc_since;
+                    const int zb = count_touching(a.bonds, a.nbr_off, a.nbr, s_kup, s_kdn, i, site);
+                    append_term(Wup, sAu, sBu, ns, nsp, nmax, nup, Kup, lup, c, wu);
+                    append_term(Wdn, sAd, sBd, ns, nsp, nmax, ndn, Kdn, ldn, c, wd);
+                    __syncthreads();
+                    if (threadIdx.x == 0) {
+                        if (can1) {
+                            s_kup[i] = 0; s_kup[site] = lup + 1;
+                            s_kdn[i] = ldn + 1; s_kdn[site] = 0;
+                        } else {
+                            s_kup[i] = lup + 1; s_kup[site] = 0;
+                            s_kdn[i] = 0; s_kdn[site] = ldn + 1;
+                        }
+                    }
+                    ++c;
+                    if (c == delay) {
+                        stream_update<T, V, DMAX>(Wup, sAu, sBu, ns, nsp, nmax, nup, c);
+                        stream_update<T, V, DMAX>(Wdn, sAd, sBd, ns, nsp, nmax, ndn, c);
+                        c = 0;
+                        ++n_flush;
+                    }
+                    __syncthreads();
+                    const int za = count_touching(a.bonds, a.nbr_off, a.nbr, s_kup, s_kdn, i, site);
+                    zmu += za - zb;
+                }
+                if (a.policy == POLICY_REFERENCE)
+                    refresh_due = (sweeps % a.nocc) == 0;
+                else
+                    refresh_due = accept && acc_since >= a.refresh_k;
+            }
+        }
+        if (a.outcomes && threadIdx.x == 0) a.outcomes[(size_t)w * a.tape_len + tidx] = (signed char)oc;
+        ++sweeps;
+        ++n_sweeps;
+        if (refresh_due) {
+            pause = true;
+            break;
+        }
+    }
+    __syncthreads();
+    for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
+        a.kup[(size_t)w * ns + t] = s_kup[t];
+        a.kdn[(size_t)w * ns + t] = s_kdn[t];
+    }
+    if (c > 0) {
+        for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) gA[t] = s_A[t];
+        for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) gB[t] = s_B[t];
+    }
+    if (threadIdx.x == 0) {
+        a.sweeps[w] = sweeps;
+        a.npend[w] = c;
+        a.acc_since[w] = acc_since;
+        a.zmu[w] = zmu;
+        if (pause) {
+            int slot = atomicAdd(a.refresh_count, 1);
+            a.refresh_list[slot] = w;
+        }
+        atomicAdd(&a.counters[CNT_MOVES], n_moves);
+        atomicAdd(&a.counters[CNT_ACCEPTED], n_acc);
+        atomicAdd(&a.counters[CNT_FLUSHES], n_flush);
+        atomicAdd(&a.counters[CNT_SWEEPS], n_sweeps);
+    }
+}
+
+size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay) {
+    const int nsp = (ns + 3) / 4 * 4;
+    const int nmax = nup > ndn ? nup : ndn;
+    const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+    return (size_t)2 * nsp * sizeof(int) + (size_t)2 * delay * (nsp + nmax) * es +
+           16 * sizeof(double);
+}
+
+template <typename T, int V, int D>
+static hipError_t launch_sweep_d(const SweepArgs& a, int nw, int threads, size_t lds,
+                                 hipStream_t s) {
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep<T, V, D>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sweep<T, V, D>), dim3(nw), dim3(threads), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int V>
+static hipError_t launch_sweep_v(const SweepArgs& a, int nw, int threads, size_t lds,
+                                 hipStream_t s) {
+    if (a.delay <= 1) return launch_sweep_d<T, V, 1>(a, nw, threads, lds, s);
+    if (a.delay <= 4) return launch_sweep_d<T, V, 4>(a, nw, threads, lds, s);
+    if (a.delay <= 8) return launch_sweep_d<T, V, 8>(a, nw, threads, lds, s);
+    return launch_sweep_d<T, V, kMaxDelay>(a, nw, threads, lds, s);
+}
+
+hipError_t launch_sweep(int prec, const SweepArgs& a, int nw, int threads, size_t lds,
+                        hipStream_t s) {
+    if (prec == 0)
+        return (a.ns % 2 == 0) ? launch_sweep_v<double, 2>(a, nw, threads, lds, s)
+                               : launch_sweep_v<double, 1>(a, nw, threads, lds, s);
+    return (a.ns % 4 == 0) ? launch_sweep_v<float, 4>(a, nw, threads, lds, s)
+                           : launch_sweep_v<float, 1>(a, nw, threads, lds, s);
+}
+
+// ---------------------------------------------------------------------------
+// Flush: applies each walker's pending rank-c update to W and clears it.
+// ---------------------------------------------------------------------------
+template <typename T, int V>
+__global__ void __launch_bounds__(1024) k_flush(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int w = a.walker0 + (int)blockIdx.x;
+    const int c = a.npend[w];
+    if (c == 0) return;
+    const int ns = a.ns, nsp = round_up(ns, 4), nmax = a.nmax, delay = a.delay;
+    T* s_A = reinterpret_cast<T*>(lds);
+    T* s_B = s_A + 2 * delay * nsp;
+    const T* gA = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * delay * nsp;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
+    for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
+    for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
+    __syncthreads();
+    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
+    T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
+    T* Wdn = Wup + (size_t)ns * a.nup;
+    stream_update<T, V, kMaxDelay>(Wup, s_A, s_B, ns, nsp, nmax, a.nup, c);
+    stream_update<T, V, kMaxDelay>(Wdn, s_A + delay * nsp, s_B + delay * nmax, ns, nsp, nmax,
+                                   a.ndn, c);
+    __syncthreads();
+    if (threadIdx.x == 0) a.npend[w] = 0;
+}
+
+hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
+    const int nsp = (a.ns + 3) / 4 * 4;
+    const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+    const size_t lds = (size_t)2 * a.delay * (nsp + a.nmax) * es;
+    const int V = prec == 0 ? 2 : 4;
+    int threads = 256;
+    const int nvec = a.ns % V == 0 ? a.ns / V : a.ns;
+    while (threads < nvec) threads += 64;
+    dim3 grid(nw), block(threads);
+    if (lds > 64 * 1024) {
+        const void* fn = prec == 0 ? (a.ns % 2 == 0 ? reinterpret_cast<const void*>(&k_flush<double, 2>)
+                                                    : reinterpret_cast<const void*>(&k_flush<double, 1>))
+                                   : (a.ns % 4 == 0 ? reinterpret_cast<const void*>(&k_flush<float, 4>)
+                                                    : reinterpret_cast<const void*>(&k_flush<float, 1>));
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (prec == 0) {
+        if (a.ns % 2 == 0) hipLaunchKernelGGL((k_flush<double, 2>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_flush<double, 1>), grid, block, lds, s, a);
+    } else {
+        if (a.ns % 4 == 0) hipLaunchKernelGGL((k_flush<float, 4>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_flush<float, 1>), grid, block, lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Refresh (reevaluateW!, src/MonteCarlo.jl:55-66).  For one spin the work
+// matrix is the ns x N "tall" matrix M = [tildeU ; U_unocc]: rows 0..N-1 are
+// U at the occupied sites ordered by label, rows N..ns-1 are U at the empty
+// sites.  Gaussian elimination with partial pivoting restricted to the top
+// N rows gives P tildeU = L1 R and L2 = U_unocc R^-1; then
+//   W_unocc = U_unocc tildeU^-1 = L2 L1^-1 P,
+// while the occupied rows of W are unit rows (W[R_l, :] = e_l).  Everything
+// is FP64; the result is rounded to the storage precision.
+// ---------------------------------------------------------------------------
+__device__ inline void atomic_max_nonneg(double* addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_refresh(RefreshArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int job = (int)blockIdx.x;
+    if (job >= a.njobs) return;
+    const int w = a.jobs[job >> 1];
+    const int spin = job & 1;
+    const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
+    const double* U = spin ? a.Udn : a.Uup;
+    const int* kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
+    double* M = a.ws + (size_t)job * ns * nmax;   // column-major, ld = ns
+    double* rv = reinterpret_cast<double*>(lds);             // [blockDim]
+    int* ri = reinterpret_cast<int*>(rv + blockDim.x);        // [blockDim]
+    int* rowsite = ri + blockDim.x;                           // [ns]
+    int* perm = rowsite + ns;                                 // [nmax]
+    __shared__ int s_bad;
+    const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
+
+    if (tid == 0) {
+        int bad = 0, cnt = N;
+        for (int l = 0; l < N; ++l) rowsite[l] = -1;
+        for (int s = 0; s < ns; ++s) {
+            int l = kap[s];
+            if (l == 0) {
+                if (cnt < ns) rowsite[cnt] = s;
+                ++cnt;
+            } else if (l < 1 || l > N || rowsite[l - 1] >= 0) {
+                bad = 1;
+            } else {
+                rowsite[l - 1] = s;
+            }
+        }
+        if (cnt != ns) bad = 1;
+        s_bad = bad;
+    }
+    for (int k = tid; k < N; k += bs) perm[k] = k;
+    __syncthreads();
+    if (s_bad) {
+        if (tid == 0) { a.status[w] = 2; atomicMin(a.failed, w); }
+        return;
+    }
+    for (int idx = tid; idx < ns * N; idx += bs) {
+        const int r = idx % ns, c = idx / ns;
+        M[idx] = U[(size_t)c * ns + rowsite[r]];
+    }
+    __syncthreads();
+
+    double logdet = 0.0, pmin = INFINITY, pmax = 0.0;
+    bool singular = false;
+    for (int k = 0; k < N; ++k) {
+        double best = -1.0;
+        int bi = N;
+        for (int r = k + tid; r < N; r += bs) {
+            double v = fabs(M[(size_t)k * ns + r]);
+            if (v > best) { best = v; bi = r; }
+        }
+        rv[tid] = best;
+        ri[tid] = bi;
+        __syncthreads();
+        for (int o = bs / 2; o > 0; o >>= 1) {
+            if (tid < o) {
+                double v2 = rv[tid + o];
+                int i2 = ri[tid + o];
+                if (v2 > rv[tid] || (v2 == rv[tid] && i2 < ri[tid])) { rv[tid] = v2; ri[tid] = i2; }
+            }
+            __syncthreads();
+        }
+        const int p = ri[0];
+        const double piv = (p < N) ? M[(size_t)k * ns + p] : 0.0;
+        __syncthreads();
+        if (!(fabs(piv) > 0.0) || !isfinite(piv)) { singular = true; break; }
+        if (p != k) {
+            for (int c = tid; c < N; c += bs) {
+                double t0 = M[(size_t)c * ns + k];
+                M[(size_t)c * ns + k] = M[(size_t)c * ns + p];
+                M[(size_t)c * ns + p] = t0;
+            }
+            if (tid == 0) { int t0 = perm[k]; perm[k] = perm[p]; perm[p] = t0; }
+        }
+        logdet += log(fabs(piv));
+        pmin = fmin(pmin, fabs(piv));
+        pmax = fmax(pmax, fabs(piv));
+        const double rp = 1.0 / piv;
+        __syncthreads();
+        for (int r = k + 1 + tid; r < ns; r += bs) M[(size_t)k * ns + r] *= rp;
+        __syncthreads();
+        const int nr = ns - k - 1, nc = N - k - 1;
+        for (int f = tid; f < nr * nc; f += bs) {
+            const int r = k + 1 + f % nr, c = k + 1 + f / nr;
+            M[(size_t)c * ns + r] -= M[(size_t)k * ns + r] * M[(size_t)c * ns + k];
+        }
+        __syncthreads();
+    }
+    if (!singular && pmin < a.pivot_tol * pmax) singular = true;
+    if (singular) {
+        if (tid == 0) { a.status[w] = 1; atomicMin(a.failed, w); }
+        return;
+    }
+    if (tid == 0) {
+        a.logdet[(size_t)w * 2 + spin] = logdet;
+        if (a.write_w && spin == 0) { a.npend[w] = 0; a.acc_since[w] = 0; }
+    }
+    // X L1 = L2, row by row (rows N..ns-1 are independent)
+    for (int r = N + tid; r < ns; r += bs) {
+        for (int c = N - 1; c >= 1; --c) {
+            const double x = M[(size_t)c * ns + r];
+            for (int c2 = 0; c2 < c; ++c2) M[(size_t)c2 * ns + r] -= x * M[(size_t)c2 * ns + c];
+        }
+    }
+    __syncthreads();
+    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
+    T* Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+    if (a.write_w) {
+        for (int idx = tid; idx < N * N; idx += bs) {
+            const int l = idx % N, c = idx / N;
+            Wo[(size_t)c * ns + rowsite[l]] = (c == l) ? T(1) : T(0);
+        }
+        const int m = ns - N;
+        for (int idx = tid; idx < m * N; idx += bs) {
+            const int r = N + idx % m, c = idx / m;
+            Wo[(size_t)perm[c] * ns + rowsite[r]] = (T)M[(size_t)c * ns + r];
+        }
+    } else {
+        // drift: compare with the current W (plus pending delayed terms)
+        const int nsp = round_up(ns, 4);
+        const int cpend = a.npend[w];
+        const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
+        const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
+        double dmax = 0.0, vmax = 0.0;
+        for (int idx = tid; idx < ns * N; idx += bs) {
+            const int r = idx % ns, c = idx / ns;
+            const int site = rowsite[r];
+            int col;
+            double vnew;
+            if (r < N) { col = c; vnew = (c == r) ? 1.0 : 0.0; }
+            else { col = perm[c]; vnew = M[(size_t)c * ns + r]; }
+            double cur = (double)Wo[(size_t)col * ns + site];
+            for (int q = 0; q < cpend; ++q) cur += (double)gA[q * nsp + site] * (double)gB[q * nmax + col];
+            dmax = fmax(dmax, fabs(vnew - cur));
+            vmax = fmax(vmax, fabs(vnew));
+        }
+        atomic_max_nonneg(&a.drift[0], dmax);
+        atomic_max_nonneg(&a.drift[1], vmax);
+    }
+}
+
+hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s) {
+    const int threads = 256;
+    const size_t lds = (size_t)(a.ns + a.nmax + 2) * sizeof(int) + (size_t)threads * (sizeof(double) + sizeof(int)) + 16;
+    dim3 grid(a.njobs), block(threads);
+    if (prec == 0) hipLaunchKernelGGL((k_refresh<double>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_refresh<float>), grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Local energy (getOL, src/Hamiltonian.jl:762-778 with spinInteraction!
+// :636-672 and SzInteraction! :593-604): one wavefront per walker, lanes
+// stride over the bond list, FP64 accumulation, shuffle reduction.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ns, int nsp,
+                                         int nmax, int K, int l, int c) {
+    double v = (double)W0[(size_t)l * ns + K];
+    for (int q = 0; q < c; ++q) v += (double)(gA[q * nsp + K] * gB[q * nmax + l]);
+    return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    const int w = (int)blockIdx.x * ((int)blockDim.x / 64) + wave;
+    if (w >= a.nw) return;
+    const int ns = a.ns, nmax = a.nmax, nsp = round_up(ns, 4);
+    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
+    const T* Wup = reinterpret_cast<const T*>(a.W) + (size_t)w * wstride;
+    const T* Wdn = Wup + (size_t)ns * a.nup;
+    const int* ku = a.kup + (size_t)w * ns;
+    const int* kd = a.kdn + (size_t)w * ns;
+    const int c = a.npend[w];
+    const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * a.delay * nsp;
+    const T* gAd = gAu + (size_t)a.delay * nsp;
+    const T* gBu = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * a.delay * nmax;
+    const T* gBd = gBu + (size_t)a.delay * nmax;
+    double diag = 0.0, off = 0.0;
+    unsigned long long nap = 0;
+    for (int b = lane; b < a.nb; b += 64) {
+        const int2 bd = a.bonds[b];
+        const int i = bd.x, j = bd.y;
+        const int iu = ku[i], ju = ku[j], id = kd[i], jd = kd[j];
+        const bool t1 = ju != 0 && id != 0;   // S+_i S-_j
+        const bool t2 = iu != 0 && jd != 0;   // S-_i S+_j
+        if (t1)
+            off += -0.5 * w_elem(Wup, gAu, gBu, ns, nsp, nmax, i, ju - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ns, nsp, nmax, j, id - 1, c);
+        if (t2)
+            off += -0.5 * w_elem(Wup, gAu, gBu, ns, nsp, nmax, j, iu - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ns, nsp, nmax, i, jd - 1, c);
+        nap += (iu != 0 && jd != 0) || (ju != 0 && id != 0);
+        const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
+        diag += szi * szj;
+    }
+    double v = diag + off;
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, 64);
+        nap += __shfl_xor(nap, o, 64);
+    }
+    if (lane == 0) {
+        a.eloc[w] = v;
+        atomicAdd(&a.counters[CNT_NAP], nap);
+    }
+}
+
+hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s) {
+    dim3 grid((a.nw + 3) / 4), block(256);
+    if (prec == 0) hipLaunchKernelGGL((k_eloc<double>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_eloc<float>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Log-derivatives.  With orthonormal orbital columns (U^T U = I, true for
+// eigenvectors of a real symmetric H), tildeU^-1 = U^T W, so
+//   O_k = tr(tildeU^-1 d tildeU) = sum_l sum_r W[r, l] Q_k[R_l, r],
+//   Q_k = dU_k U^T (ns x ns, built once per ansatz).
+// One workgroup per walker; wavefronts take labels, lanes stride over r.
+// Requires W without pending terms (the host flushes first).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_grad(ElocArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int w = (int)blockIdx.x;
+    const int ns = a.ns, nmax = a.nmax, P = a.nparams;
+    int* Rsite = reinterpret_cast<int*>(lds);           // [2][nmax]
+    double* acc = reinterpret_cast<double*>(Rsite + 2 * nmax + 2);  // [P] (aligned enough)
+    const int tid = (int)threadIdx.x;
+    for (int s = tid; s < ns; s += (int)blockDim.x) {
+        const int lu = a.kup[(size_t)w * ns + s], ld = a.kdn[(size_t)w * ns + s];
+        if (lu) Rsite[lu - 1] = s;
+        if (ld) Rsite[nmax + ld - 1] = s;
+    }
+    for (int p = tid; p < P; p += (int)blockDim.x) acc[p] = 0.0;
+    __syncthreads();
+    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
+    const int wave = tid / 64, lane = tid % 64, nwave = (int)blockDim.x / 64;
+    for (int spin = 0; spin < 2; ++spin) {
+        const int N = spin ? a.ndn : a.nup;
+        const T* W = reinterpret_cast<const T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+        const double* Q = spin ? a.Qdn : a.Qup;
+        for (int p = 0; p < P; ++p) {
+            const double* Qp = Q + (size_t)p * ns * ns;
+            double part = 0.0;
+            for (int l = wave; l < N; l += nwave) {
+                const double* qrow = Qp + (size_t)Rsite[spin * nmax + l] * ns;
+                const T* wcol = W + (size_t)l * ns;
+                for (int r = lane; r < ns; r += 64) part += (double)wcol[r] * qrow[r];
+            }
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+            if (lane == 0) atomicAdd(&acc[p], part);
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < P; p += (int)blockDim.x) a.O[(size_t)w * P + p] = acc[p];
+}
+
+hipError_t launch_grad(int prec, const ElocArgs& a, hipStream_t s) {
+    const size_t lds = (size_t)(2 * a.nmax + 2) * sizeof(int) + (size_t)a.nparams * sizeof(double) + 16;
+    dim3 grid(a.nw), block(256);
+    if (prec == 0) hipLaunchKernelGGL((k_grad<double>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_grad<float>), grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+// Q_k[s][r] = sum_m dU_k[s, m] U[r, m]  (row-major ns x ns per parameter).
+__global__ void k_q_build(const double* __restrict__ dU, const double* __restrict__ U, int ns,
+                          int n, int nparams, double* __restrict__ Q) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nparams * ns * ns;
+    if (idx >= total) return;
+    const int p = (int)(idx / ((long long)ns * ns));
+    const int rem = (int)(idx % ((long long)ns * ns));
+    const int s = rem / ns, r = rem % ns;
+    const double* d = dU + (size_t)p * ns * n;
+    double v = 0.0;
+    for (int m = 0; m < n; ++m) v += d[(size_t)m * ns + s] * U[(size_t)m * ns + r];
+    Q[idx] = v;
+}
+
+hipError_t launch_q_build(const double* dU, const double* U, int ns, int n, int nparams,
+                          double* Q, hipStream_t s) {
+    const long long total = (long long)nparams * ns * ns;
+    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_q_build, grid, block, 0, s, dU, U, ns, n, nparams, Q);
+    return hipGetLastError();
+}
+
+}  // namespace kvmc

@@ -1,0 +1,135 @@
+// Internal declarations shared by the device code (kvmc_device.hip) and the
+// C-ABI host layer (kvmc_api.hip).  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kvmc {
+
+// Per-sweep outcome codes (mirror of KVMC_OUT_* in kagome_vmc.h).
+enum : int { OUT_PRE_REJECT = 0, OUT_NO_MOVE = 1, OUT_REJECT = 2, OUT_ACCEPT = 3 };
+enum : int { KERNEL_REFERENCE = 0, KERNEL_UNBIASED = 1 };
+enum : int { POLICY_REFERENCE = 0, POLICY_EVERY_K = 1 };
+
+// Device-wide counters (u64 slots).
+enum : int { CNT_MOVES = 0, CNT_ACCEPTED = 1, CNT_FLUSHES = 2, CNT_SWEEPS = 3, CNT_NAP = 4,
+             CNT_SLOTS = 8 };
+
+// Largest number of pending rank-1 updates kept per spin before a flush.
+constexpr int kMaxDelay = 16;
+
+// Philox4x32-10 (Salmon et al., SC'11): counter-based, so every
+// (walker, sweep) draw is addressable without carried state.
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Sweep draws of (seed, walker, sweep counter): u in [0,1) from 53 bits and
+// a bond index in [0, nb) by 32x32 multiply-high.
+__host__ __device__ inline void sweep_draws(uint64_t seed, uint32_t walker, int64_t sweep, int nb,
+                                            double* u, int* bond) {
+    uint32_t x[4];
+    philox4x32_10((uint32_t)sweep, (uint32_t)((uint64_t)sweep >> 32), walker, 0u, (uint32_t)seed,
+                  (uint32_t)(seed >> 32), x);
+    *u = ((double)(x[0] >> 5) * 67108864.0 + (double)(x[1] >> 6)) * (1.0 / 9007199254740992.0);
+    *bond = (int)(((uint64_t)x[2] * (uint64_t)(uint32_t)nb) >> 32);
+}
+
+// Arguments of the sweep kernel.  Every pointer is device memory.
+struct SweepArgs {
+    int ns, nup, ndn, nb, nocc;     // nocc = min(nup, ndn)
+    const int2* bonds;              // nb pairs, 0-based
+    const int* nbr_off;             // ns+1 offsets into nbr
+    const int* nbr;                 // bond ids touching each site
+    int* kup;                       // [nw][ns] labels 1-based, 0 empty
+    int* kdn;
+    void* W;                        // [nw][ns*(nup+ndn)] column-major per spin
+    void* A;                        // [nw][2][delay][ns]  pending columns
+    void* Bm;                       // [nw][2][delay][nmax] pending scaled rows
+    int* npend;                     // [nw] pending count
+    int delay;                      // 1 = immediate rank-1 update
+    int nmax;                       // max(nup, ndn)
+    long long* sweeps;              // [nw] Carlo ctx.sweeps
+    const long long* stop;          // [nw] sweep count at which this call ends
+    const long long* tape_base;     // [nw] sweep count at the start of the call
+    int* zmu;                       // [nw] antiparallel bond count
+    int* acc_since;                 // [nw] accepted moves since last refresh
+    int* refresh_list;              // walkers paused for a refresh
+    int* refresh_count;
+    unsigned long long* counters;   // CNT_*
+    uint64_t seed;
+    const double* tape_u;           // [nw][tape_len] or null
+    const int* tape_bond;
+    long long tape_len;
+    signed char* outcomes;          // [nw][tape_len] or null
+    int kernel_mode, policy, refresh_k;
+    int walker0;                    // first walker of this launch
+};
+
+struct RefreshArgs {
+    int ns, nup, ndn;
+    const double* Uup;              // ns x nup column-major
+    const double* Udn;
+    const int* kup;                 // [nw][ns]
+    const int* kdn;
+    void* W;                        // [nw][ns*(nup+ndn)]
+    const void* A;                  // pending (drift mode reads W + A B^T)
+    const void* Bm;
+    int* npend;                     // cleared for refreshed walkers (write_w)
+    int* acc_since;                 // cleared for refreshed walkers (write_w)
+    int delay, nmax;
+    const int* jobs;                // walker ids; job j -> walker jobs[j/2], spin j%2
+    int njobs;
+    double* ws;                     // [njobs][ns*nmax] FP64 work matrices
+    double* logdet;                 // [nw][2]
+    int* status;                    // [nw] 0 ok, 1 singular
+    int* failed;                    // smallest failing walker (atomicMin)
+    int write_w;                    // 1: replace W; 0: measure drift only
+    double pivot_tol;               // reject when min|pivot| < pivot_tol * max|pivot|
+    double* drift;                  // [2]: max |W_new - W_cur|, max |W_new| (as ordered bits)
+};
+
+struct ElocArgs {
+    int ns, nup, ndn, nb;
+    const int2* bonds;
+    const int* kup;
+    const int* kdn;
+    const void* W;
+    const void* A;
+    const void* Bm;
+    const int* npend;
+    int delay, nmax;
+    int nw;
+    double* eloc;                   // [nw]
+    unsigned long long* counters;   // CNT_NAP accumulates antiparallel counts
+    // log-derivatives: Q_k = dU_k U^T (ns x ns row-major per parameter and spin)
+    int nparams;
+    const double* Qup;              // [nparams][ns][ns]
+    const double* Qdn;
+    double* O;                      // [nw][nparams]
+};
+
+// Launchers (defined in kvmc_device.hip).  `prec` 0 = FP64, 1 = FP32.
+hipError_t launch_sweep(int prec, const SweepArgs& a, int nwalkers, int threads, size_t lds,
+                        hipStream_t s);
+size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay);
+hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s);
+hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s);
+hipError_t launch_grad(int prec, const ElocArgs& a, hipStream_t s);
+hipError_t launch_flush(int prec, const SweepArgs& a, int nwalkers, hipStream_t s);
+hipError_t launch_q_build(const double* dU, const double* U, int ns, int n, int nparams,
+                          double* Q, hipStream_t s);
+
+}  // namespace kvmc

@@ -1,0 +1,108 @@
+"""Host setup of the product (kagome_vmc.lattice / .hamiltonian): the
+reference's known answers and agreement with the oracle's independent
+restatement of Hmat / get_nn."""
+import numpy as np
+import pytest
+
+from kagome_vmc import (DoubleKagome, Hamiltonian, Hmat, get_boundary_shifts, get_nn,
+                        get_site_coord, ns, orbitals, shell_gap, unitcell_coord)
+
+
+def test_lattice_known_answers():
+    with pytest.raises(AssertionError):
+        DoubleKagome.build(1.0, 3, 3, (False, False))
+    assert ns(DoubleKagome.build(1.0, 4, 3, (False, False))) == 36
+    with pytest.raises(ValueError):
+        DoubleKagome.build(1.0, 4, 3, (False, False), (True, False))
+    lat = DoubleKagome.build(1.0, 4, 3, (False, False))
+    assert np.allclose(get_site_coord(lat, 3), [0.5, 0.5 * np.sqrt(3)])
+    assert np.allclose(get_site_coord(lat, 13), [1.0, np.sqrt(3)])
+    assert np.allclose(unitcell_coord(DoubleKagome.build(1.0, 4, 2, (False, False)), 24),
+                       [5.0, np.sqrt(3)])
+    lp = DoubleKagome.build(1.0, 4, 3, (True, True))
+    sh = get_boundary_shifts(lp, 3, 9)
+    assert (1, 0, 1.0) in sh
+    with pytest.raises(AssertionError):
+        get_boundary_shifts(lp, 3, 3)
+
+
+def test_hmat_known_answers():
+    H = Hmat(DoubleKagome.build(1.0, 4, 3, (False, False)))
+    assert H[0, 1] == -1 and H[0, 2] == -1 and np.isclose(H[2, 12], 1)
+    assert np.allclose(H[0, 3:], 0)
+    H = Hmat(DoubleKagome.build(1.0, 4, 3, (True, True), (True, True)))
+    assert H[0, 10] == 1 and H[0, 26] == -1 and H[10, 26] == 1
+    H = Hmat(DoubleKagome.build(1.0, 4, 3, (True, True), (False, True)))
+    assert H[0, 10] == -1 and H[0, 26] == -1
+
+
+@pytest.mark.parametrize("n1,n2", [(2, 2), (4, 3), (4, 4), (6, 6)])
+@pytest.mark.parametrize("pbc,apbc", [((False, False), (False, False)),
+                                      ((True, False), (True, False)),
+                                      ((True, True), (False, True)),
+                                      ((True, True), (True, True))])
+@pytest.mark.parametrize("B", [0.0, 0.1])
+def test_hmat_matches_oracle(oracle_mod, n1, n2, pbc, apbc, B):
+    Hp = Hmat(DoubleKagome.build(1.0, n1, n2, pbc, apbc), B=B)
+    Ho = oracle_mod.OracleLattice(1.0, n1, n2, pbc, apbc).hmat(B=B)
+    assert np.abs(Hp - Ho).max() < 1e-13
+    assert get_nn(Hp) == oracle_mod.get_nn(Ho)
+
+
+@pytest.mark.parametrize("L,apbc,closed", [(4, (False, True), True), (4, (False, False), False),
+                                           (6, (False, False), True), (8, (False, True), True),
+                                           (12, (False, True), True)])
+def test_shell_closure(L, apbc, closed):
+    """Closed shells (read off the spectrum) decide the boundary conditions:
+    antiperiodic along a2 for L = 4, 8, 12, plain periodic for L = 6."""
+    H = Hmat(DoubleKagome.build(1.0, L, L, (True, True), apbc))
+    _, _, w = orbitals(H, 3 * L * L // 2, 3 * L * L // 2)
+    assert (shell_gap(w, 3 * L * L // 2) > 1e-6) == closed
+
+
+def test_pi_flux_pattern():
+    """Flux 0 through every triangle and pi through every hexagon: the product
+    of hoppings around each 3-cycle is positive (t = -H) and around each
+    elementary 6-cycle negative, for the reference's pi-flux tables."""
+    H = Hmat(DoubleKagome.build(1.0, 8, 8, (True, True), (False, False)))
+    T = -H.real
+    n = T.shape[0]
+    nbrs = [set(np.nonzero(T[i])[0]) for i in range(n)]
+    tri = set()
+    for i in range(n):
+        for j in nbrs[i]:
+            for k in nbrs[j] & nbrs[i]:
+                tri.add(tuple(sorted((i, j, k))))
+    assert len(tri) == 2 * n // 3
+    for (i, j, k) in tri:
+        assert T[i, j] * T[j, k] * T[k, i] > 0
+    # hexagons: 6-cycles with no chord, found by walking neighbours
+    hexes = set()
+    for a in range(n):
+        for b in nbrs[a]:
+            for c in nbrs[b] - {a}:
+                if c in nbrs[a]:
+                    continue
+                for d in nbrs[c] - {a, b}:
+                    if d in nbrs[b]:
+                        continue
+                    for e in nbrs[d] - {a, b, c}:
+                        if e in nbrs[c]:
+                            continue
+                        for f in (nbrs[e] & nbrs[a]) - {b, c, d}:
+                            if f in nbrs[d] or f in nbrs[b]:
+                                continue
+                            cyc = (a, b, c, d, e, f)
+                            hexes.add(frozenset(cyc))
+                            prod = T[a, b] * T[b, c] * T[c, d] * T[d, e] * T[e, f] * T[f, a]
+                            assert prod < 0
+    assert len(hexes) == n // 3
+
+
+def test_hamiltonian_bundle():
+    lat = DoubleKagome.build(1.0, 4, 3, (True, False))
+    ham = Hamiltonian.build(18, 18, lat)
+    assert ham.U_up.shape == (36, 18) and ham.U_down.shape == (36, 18)
+    assert np.allclose(ham.U_up.T @ ham.U_up, np.eye(18), atol=1e-12)
+    assert np.allclose(ham.H_mat @ ham.U_up, ham.U_up * ham.spectrum[:18], atol=1e-12)
+    assert ham.bonds_array().shape == (len(ham.nn), 2)

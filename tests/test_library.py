@@ -1,0 +1,64 @@
+"""The C-ABI library loads and exports every entry point the header declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kagome_vmc.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(kvmc_[a-z_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from kagome_vmc import _native
+    if not os.path.exists(_native.LIB_PATH):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location(
+            "kvmc_build", os.path.join(ROOT, "kagomedsl.jl_amd", "build.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod.build()
+    return _native.load()
+
+
+def test_header_declares_the_abi():
+    syms = declared_symbols()
+    for s in ["kvmc_create", "kvmc_sweep", "kvmc_measure", "kvmc_refresh", "kvmc_destroy"]:
+        assert s in syms
+
+
+def test_every_declared_symbol_is_exported(lib):
+    from kagome_vmc import _native
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+    assert sorted(_native.EXPORTS) == declared_symbols()
+
+
+def test_version_and_error_string(lib):
+    assert b"gfx950" in lib.kvmc_version()
+    assert isinstance(lib.kvmc_last_error(), bytes)
+
+
+def test_struct_layout_matches_header(lib):
+    from kagome_vmc import _native
+    # kvmc_desc: 4 int32, 3 pointers, int64, 6 int32, uint64 (x86-64 layout)
+    assert ctypes.sizeof(_native.KvmcDesc) == 4 * 4 + 3 * 8 + 8 + 6 * 4 + 8
+    assert ctypes.sizeof(_native.KvmcCounters) == 6 * 8
+
+
+def test_invalid_descriptor_is_rejected_before_any_gpu_work(lib):
+    """Argument validation happens on the host before HIP is touched."""
+    from kagome_vmc import _native
+    d = _native.KvmcDesc()
+    d.ns, d.n_up, d.n_down, d.nb = 12, 5, 6, 0     # n_up + n_down != ns
+    h = ctypes.c_void_p()
+    rc = lib.kvmc_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _native.KVMC_ERR_INVALID
+    assert b"n_up + n_down" in lib.kvmc_last_error()
