@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "kagome_vmc", "libkagome_vmc.so")
-SOURCES = ["kvmc_device.hip", "kvmc_api.hip"]
+SOURCES = ["kvmc_device.hip", "kvmc_refresh.hip", "kvmc_api.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off: plain v_fma_f32 only

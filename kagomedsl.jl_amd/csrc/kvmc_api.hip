@@ -109,6 +109,8 @@ struct kvmc_handle {
     int* nbr = nullptr;
     double* Uup = nullptr;
     double* Udn = nullptr;
+    double* UTup = nullptr;   // row-major copies for the refresh gathers
+    double* UTdn = nullptr;
     int* kup = nullptr;
     int* kdn = nullptr;
     void* W = nullptr;
@@ -144,7 +146,7 @@ struct kvmc_handle {
 namespace {
 
 void free_all(kvmc_handle* h) {
-    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->kup, h->kdn, h->W, h->A,
+    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->kup, h->kdn, h->W, h->A,
                     h->Bm, h->npend, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
                     h->tbase, h->rlist, h->rcount, h->failed, h->all_walkers, h->counters,
                     h->logdet, h->eloc, h->drift, h->ws, h->Qup, h->Qdn, h->O};
@@ -194,6 +196,7 @@ int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
     HIP_TRY(hipMemcpyAsync(h->failed, &kNone, sizeof(int), hipMemcpyHostToDevice, h->stream));
     RefreshArgs r{};
     r.ns = h->ns; r.nup = h->nup; r.ndn = h->ndn; r.Uup = h->Uup; r.Udn = h->Udn;
+    r.UTup = h->UTup; r.UTdn = h->UTdn;
     r.kup = h->kup; r.kdn = h->kdn; r.W = h->W; r.A = h->A; r.Bm = h->Bm;
     r.npend = h->npend; r.acc_since = h->acc_since; r.delay = h->delay; r.nmax = h->nmax;
     r.ws = h->ws; r.logdet = h->logdet; r.status = h->status; r.failed = h->failed;
@@ -294,6 +297,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     if (delay > kMaxDelay)
         return fail(KVMC_ERR_INVALID, "delay must be <= " + std::to_string(kMaxDelay));
     if (d->ns > 4096) return fail(KVMC_ERR_INVALID, "ns > 4096 is not supported");
+    if (!refresh_supported(d->ns, d->n_up, d->n_down))
+        return fail(KVMC_ERR_INVALID, "lattice too large for the refresh kernel (LDS / rows per thread)");
     for (int b = 0; b < d->nb; ++b) {
         int i = d->bonds[2 * b], j = d->bonds[2 * b + 1];
         if (i < 1 || j < 1 || i > d->ns || j > d->ns || i > j)
@@ -318,6 +323,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     ALLOC(h->nbr, (size_t)2 * h->nb);
     ALLOC(h->Uup, ns * h->nup);
     ALLOC(h->Udn, ns * h->ndn);
+    ALLOC(h->UTup, ns * h->nup);
+    ALLOC(h->UTdn, ns * h->ndn);
     ALLOC(h->kup, nw * ns);
     ALLOC(h->kdn, nw * ns);
     {
@@ -333,12 +340,12 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     ALLOC(h->logdet, 2 * nw); ALLOC(h->eloc, nw); ALLOC(h->drift, 2);
     // refresh workspace: up to ~1 GiB of FP64 work matrices per batch
     {
-        const size_t per_job = ns * h->nmax * sizeof(double);
+        const size_t per_job = refresh_ws_doubles_per_job(h->ns, h->nmax) * sizeof(double);
         size_t jobs = std::max<size_t>(2, ((size_t)1 << 30) / per_job);
         jobs = std::min(jobs, 2 * nw);
         h->ws_jobs = (int)(jobs & ~(size_t)1);
         if (h->ws_jobs < 2) h->ws_jobs = 2;
-        ALLOC(h->ws, (size_t)h->ws_jobs * ns * h->nmax);
+        ALLOC(h->ws, (size_t)h->ws_jobs * refresh_ws_doubles_per_job(h->ns, h->nmax));
     }
 #undef ALLOC
     // bonds (0-based) and site -> bond adjacency
@@ -354,6 +361,11 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         off[s + 1] = off[s] + (int)adj[s].size();
         lst.insert(lst.end(), adj[s].begin(), adj[s].end());
     }
+    std::vector<double> utu(ns * h->nup), utd(ns * h->ndn);
+    for (size_t s2 = 0; s2 < ns; ++s2) {
+        for (int c = 0; c < h->nup; ++c) utu[s2 * h->nup + c] = d->u_up[(size_t)c * ns + s2];
+        for (int c = 0; c < h->ndn; ++c) utd[s2 * h->ndn + c] = d->u_down[(size_t)c * ns + s2];
+    }
     std::vector<int> allw(nw);
     for (size_t w = 0; w < nw; ++w) allw[w] = (int)w;
     auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -364,6 +376,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         (lst.size() && up(h->nbr, lst.data(), lst.size() * sizeof(int)) != hipSuccess) ||
         up(h->Uup, d->u_up, ns * h->nup * sizeof(double)) != hipSuccess ||
         up(h->Udn, d->u_down, ns * h->ndn * sizeof(double)) != hipSuccess ||
+        up(h->UTup, utu.data(), utu.size() * sizeof(double)) != hipSuccess ||
+        up(h->UTdn, utd.data(), utd.size() * sizeof(double)) != hipSuccess ||
         up(h->all_walkers, allw.data(), nw * sizeof(int)) != hipSuccess ||
         hipMemsetAsync(h->sweeps, 0, nw * sizeof(long long), h->stream) != hipSuccess ||
         hipMemsetAsync(h->npend, 0, nw * sizeof(int), h->stream) != hipSuccess ||
@@ -437,11 +451,10 @@ int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_
             return fail(KVMC_ERR_SINGULAR, std::to_string(bad.size()) +
                                                " walkers still singular after " +
                                                std::to_string(max_attempts) + " draws");
-        for (int w : bad) {
+        for (int w : bad)
             random_config(seed, (uint32_t)w, (uint32_t)attempt, h->ns, h->nup, h->ndn, &ku[w * ns], &kd[w * ns]);
-            HIP_TRY(hipMemcpyAsync(h->kup + w * ns, &ku[w * ns], ns * sizeof(int), hipMemcpyHostToDevice, h->stream));
-            HIP_TRY(hipMemcpyAsync(h->kdn + w * ns, &kd[w * ns], ns * sizeof(int), hipMemcpyHostToDevice, h->stream));
-        }
+        HIP_TRY(hipMemcpyAsync(h->kup, ku.data(), ku.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->kdn, kd.data(), kd.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
         nredraw += (int64_t)bad.size();
         int* dbad = nullptr;
         if ((rc = dalloc(&dbad, bad.size())) != KVMC_OK) return rc;

@@ -6,8 +6,7 @@
 //              (columns A_q, scaled rows B_q in LDS) and applied to W in HBM
 //              by one streaming rank-`delay` pass; delay = 1 is the
 //              reference's immediate geru (src/MonteCarlo.jl:279-292).
-//   k_refresh  FP64 pivoted LU of tildeU and W = U tildeU^-1
-//              (src/MonteCarlo.jl:55-115), one workgroup per (walker, spin).
+//   (the FP64 W refresh lives in kvmc_refresh.hip)
 //   k_eloc     Heisenberg J1 local energy (src/Hamiltonian.jl:762-778), one
 //              wavefront per walker, FP64 accumulation.
 //   k_grad     log-derivatives O_k = d log det tildeU / d theta_k.
@@ -399,175 +398,6 @@ hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
         if (a.ns % 4 == 0) hipLaunchKernelGGL((k_flush<float, 4>), grid, block, lds, s, a);
         else hipLaunchKernelGGL((k_flush<float, 1>), grid, block, lds, s, a);
     }
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// Refresh (reevaluateW!, src/MonteCarlo.jl:55-66).  For one spin the work
-// matrix is the ns x N "tall" matrix M = [tildeU ; U_unocc]: rows 0..N-1 are
-// U at the occupied sites ordered by label, rows N..ns-1 are U at the empty
-// sites.  Gaussian elimination with partial pivoting restricted to the top
-// N rows gives P tildeU = L1 R and L2 = U_unocc R^-1; then
-//   W_unocc = U_unocc tildeU^-1 = L2 L1^-1 P,
-// while the occupied rows of W are unit rows (W[R_l, :] = e_l).  Everything
-// is FP64; the result is rounded to the storage precision.
-// ---------------------------------------------------------------------------
-__device__ inline void atomic_max_nonneg(double* addr, double v) {
-    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_refresh(RefreshArgs a) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const int job = (int)blockIdx.x;
-    if (job >= a.njobs) return;
-    const int w = a.jobs[job >> 1];
-    const int spin = job & 1;
-    const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
-    const double* U = spin ? a.Udn : a.Uup;
-    const int* kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
-    double* M = a.ws + (size_t)job * ns * nmax;   // column-major, ld = ns
-    double* rv = reinterpret_cast<double*>(lds);             // [blockDim]
-    int* ri = reinterpret_cast<int*>(rv + blockDim.x);        // [blockDim]
-    int* rowsite = ri + blockDim.x;                           // [ns]
-    int* perm = rowsite + ns;                                 // [nmax]
-    __shared__ int s_bad;
-    const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
-
-    if (tid == 0) {
-        int bad = 0, cnt = N;
-        for (int l = 0; l < N; ++l) rowsite[l] = -1;
-        for (int s = 0; s < ns; ++s) {
-            int l = kap[s];
-            if (l == 0) {
-                if (cnt < ns) rowsite[cnt] = s;
-                ++cnt;
-            } else if (l < 1 || l > N || rowsite[l - 1] >= 0) {
-                bad = 1;
-            } else {
-                rowsite[l - 1] = s;
-            }
-        }
-        if (cnt != ns) bad = 1;
-        s_bad = bad;
-    }
-    for (int k = tid; k < N; k += bs) perm[k] = k;
-    __syncthreads();
-    if (s_bad) {
-        if (tid == 0) { a.status[w] = 2; atomicMin(a.failed, w); }
-        return;
-    }
-    for (int idx = tid; idx < ns * N; idx += bs) {
-        const int r = idx % ns, c = idx / ns;
-        M[idx] = U[(size_t)c * ns + rowsite[r]];
-    }
-    __syncthreads();
-
-    double logdet = 0.0, pmin = INFINITY, pmax = 0.0;
-    bool singular = false;
-    for (int k = 0; k < N; ++k) {
-        double best = -1.0;
-        int bi = N;
-        for (int r = k + tid; r < N; r += bs) {
-            double v = fabs(M[(size_t)k * ns + r]);
-            if (v > best) { best = v; bi = r; }
-        }
-        rv[tid] = best;
-        ri[tid] = bi;
-        __syncthreads();
-        for (int o = bs / 2; o > 0; o >>= 1) {
-            if (tid < o) {
-                double v2 = rv[tid + o];
-                int i2 = ri[tid + o];
-                if (v2 > rv[tid] || (v2 == rv[tid] && i2 < ri[tid])) { rv[tid] = v2; ri[tid] = i2; }
-            }
-            __syncthreads();
-        }
-        const int p = ri[0];
-        const double piv = (p < N) ? M[(size_t)k * ns + p] : 0.0;
-        __syncthreads();
-        if (!(fabs(piv) > 0.0) || !isfinite(piv)) { singular = true; break; }
-        if (p != k) {
-            for (int c = tid; c < N; c += bs) {
-                double t0 = M[(size_t)c * ns + k];
-                M[(size_t)c * ns + k] = M[(size_t)c * ns + p];
-                M[(size_t)c * ns + p] = t0;
-            }
-            if (tid == 0) { int t0 = perm[k]; perm[k] = perm[p]; perm[p] = t0; }
-        }
-        logdet += log(fabs(piv));
-        pmin = fmin(pmin, fabs(piv));
-        pmax = fmax(pmax, fabs(piv));
-        const double rp = 1.0 / piv;
-        __syncthreads();
-        for (int r = k + 1 + tid; r < ns; r += bs) M[(size_t)k * ns + r] *= rp;
-        __syncthreads();
-        const int nr = ns - k - 1, nc = N - k - 1;
-        for (int f = tid; f < nr * nc; f += bs) {
-            const int r = k + 1 + f % nr, c = k + 1 + f / nr;
-            M[(size_t)c * ns + r] -= M[(size_t)k * ns + r] * M[(size_t)c * ns + k];
-        }
-        __syncthreads();
-    }
-    if (!singular && pmin < a.pivot_tol * pmax) singular = true;
-    if (singular) {
-        if (tid == 0) { a.status[w] = 1; atomicMin(a.failed, w); }
-        return;
-    }
-    if (tid == 0) {
-        a.logdet[(size_t)w * 2 + spin] = logdet;
-        if (a.write_w && spin == 0) { a.npend[w] = 0; a.acc_since[w] = 0; }
-    }
-    // X L1 = L2, row by row (rows N..ns-1 are independent)
-    for (int r = N + tid; r < ns; r += bs) {
-        for (int c = N - 1; c >= 1; --c) {
-            const double x = M[(size_t)c * ns + r];
-            for (int c2 = 0; c2 < c; ++c2) M[(size_t)c2 * ns + r] -= x * M[(size_t)c2 * ns + c];
-        }
-    }
-    __syncthreads();
-    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
-    T* Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
-    if (a.write_w) {
-        for (int idx = tid; idx < N * N; idx += bs) {
-            const int l = idx % N, c = idx / N;
-            Wo[(size_t)c * ns + rowsite[l]] = (c == l) ? T(1) : T(0);
-        }
-        const int m = ns - N;
-        for (int idx = tid; idx < m * N; idx += bs) {
-            const int r = N + idx % m, c = idx / m;
-            Wo[(size_t)perm[c] * ns + rowsite[r]] = (T)M[(size_t)c * ns + r];
-        }
-    } else {
-        // drift: compare with the current W (plus pending delayed terms)
-        const int nsp = round_up(ns, 4);
-        const int cpend = a.npend[w];
-        const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
-        const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
-        double dmax = 0.0, vmax = 0.0;
-        for (int idx = tid; idx < ns * N; idx += bs) {
-            const int r = idx % ns, c = idx / ns;
-            const int site = rowsite[r];
-            int col;
-            double vnew;
-            if (r < N) { col = c; vnew = (c == r) ? 1.0 : 0.0; }
-            else { col = perm[c]; vnew = M[(size_t)c * ns + r]; }
-            double cur = (double)Wo[(size_t)col * ns + site];
-            for (int q = 0; q < cpend; ++q) cur += (double)gA[q * nsp + site] * (double)gB[q * nmax + col];
-            dmax = fmax(dmax, fabs(vnew - cur));
-            vmax = fmax(vmax, fabs(vnew));
-        }
-        atomic_max_nonneg(&a.drift[0], dmax);
-        atomic_max_nonneg(&a.drift[1], vmax);
-    }
-}
-
-hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s) {
-    const int threads = 256;
-    const size_t lds = (size_t)(a.ns + a.nmax + 2) * sizeof(int) + (size_t)threads * (sizeof(double) + sizeof(int)) + 16;
-    dim3 grid(a.njobs), block(threads);
-    if (prec == 0) hipLaunchKernelGGL((k_refresh<double>), grid, block, lds, s, a);
-    else hipLaunchKernelGGL((k_refresh<float>), grid, block, lds, s, a);
     return hipGetLastError();
 }
 

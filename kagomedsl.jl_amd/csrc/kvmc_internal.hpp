@@ -82,6 +82,8 @@ struct RefreshArgs {
     int ns, nup, ndn;
     const double* Uup;              // ns x nup column-major
     const double* Udn;
+    const double* UTup;             // ns x nup row-major copy of Uup
+    const double* UTdn;
     const int* kup;                 // [nw][ns]
     const int* kdn;
     void* W;                        // [nw][ns*(nup+ndn)]
@@ -92,7 +94,7 @@ struct RefreshArgs {
     int delay, nmax;
     const int* jobs;                // walker ids; job j -> walker jobs[j/2], spin j%2
     int njobs;
-    double* ws;                     // [njobs][ns*nmax] FP64 work matrices
+    double* ws;                     // [njobs][refresh_ws_doubles_per_job] FP64 work matrices
     double* logdet;                 // [nw][2]
     int* status;                    // [nw] 0 ok, 1 singular
     int* failed;                    // smallest failing walker (atomicMin)
@@ -126,6 +128,8 @@ hipError_t launch_sweep(int prec, const SweepArgs& a, int nwalkers, int threads,
                         hipStream_t s);
 size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay);
 hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s);
+size_t refresh_ws_doubles_per_job(int ns, int nmax);
+bool refresh_supported(int ns, int nup, int ndn);
 hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s);
 hipError_t launch_grad(int prec, const ElocArgs& a, hipStream_t s);
 hipError_t launch_flush(int prec, const SweepArgs& a, int nwalkers, hipStream_t s);

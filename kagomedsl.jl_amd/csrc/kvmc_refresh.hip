@@ -1,0 +1,629 @@
+// W refresh (reevaluateW!, src/MonteCarlo.jl:55-66) as a blocked FP64
+// factorisation on gfx950: one 512-thread workgroup per (walker, spin).
+//
+// Work matrix (row-major, zero padded): M = [ tildeU ; U_empty ], ns x N.
+// Physical rows 0..N-1 are U at the occupied sites ordered by label (tildeU,
+// as in tilde_U, src/MonteCarlo.jl:92-115); rows N..ns-1 are U at the empty
+// sites.  Row interchanges are implicit: phys[] maps the logical (pivoted)
+// row order onto physical rows, so no row of M ever moves.
+//   1. Right-looking LU with partial pivoting restricted to rows < N, panels
+//      of RB = 32 columns.  The panel lives in registers (one row per
+//      thread); per pivot column one barrier publishes each wavefront's best
+//      candidate row.  U12 = L11^-1 A12 and the trailing update both run on
+//      FP64 MFMA (v_mfma_f64_16x16x4f64), L11^-1 being the explicit inverse
+//      of the unit-lower 32x32 block.  Logical rows >= N end as
+//      L2 = U_empty R^-1.
+//   2. X L1 = L2 (unit lower, right side), panels right to left: X_J = B_J
+//      L1_JJ^-1 and the update of the remaining columns, both on MFMA.
+//   3. W_empty = X P; occupied rows of W are unit rows.  Written coalesced per
+//      label column, rounded to the storage precision.
+#include "kvmc_internal.hpp"
+
+#include <math.h>
+
+namespace kvmc {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4f64 operand layout (lane l of a wavefront):
+//   A[i][k] with i = l % 16, k = l / 16;  B[k][j] with k = l / 16, j = l % 16;
+//   D register g holds D[l / 16 + 4 g][l % 16].
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int RB = 32;    // panel width
+constexpr int RT = 256;   // threads per workgroup
+constexpr int NWAVE = RT / 64;
+constexpr int LB = RB + 1;  // LDS row stride of 32-wide blocks
+
+__host__ __device__ inline int rup_i(int x, int m) { return (x + m - 1) / m * m; }
+__host__ __device__ inline int refresh_nrows(int ns) { return rup_i(ns, 16) + 16; }
+__host__ __device__ inline int refresh_ldm(int nmax) { return rup_i(nmax, 32); }
+// Work doubles per job: the padded tall matrix plus the saved L11 inverses.
+__host__ __device__ inline size_t refresh_ws_job(int ns, int nmax) {
+    return (size_t)refresh_nrows(ns) * refresh_ldm(nmax) + (size_t)((nmax + RB - 1) / RB) * RB * RB;
+}
+
+__device__ inline void atomic_max_nonneg2(double* addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(addr),
+              (unsigned long long)__double_as_longlong(v));
+}
+
+// Inverse of the unit-lower jb x jb block L (row stride LB, entries above the
+// diagonal ignored) into Linv (row stride LB), identity beyond jb.  Lanes
+// 0..RB-1 of wavefront 0 each hold one column of the inverse in registers and
+// run its forward substitution fully unrolled (four partial sums per row to
+// shorten the dependency chain); L is read as LDS broadcasts.
+__device__ __attribute__((noinline)) void unit_lower_inverse(const double* L, double* Linv, int jb) {
+    if (threadIdx.x >= 64) return;
+    const int j = (int)threadIdx.x & (RB - 1);
+    double x[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        double v = (i == j) ? 1.0 : 0.0;
+        if (i > j && i < jb) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+            for (int k = 0; k < i; k += 4) {
+                s0 += L[i * LB + k] * x[k];
+                if (k + 1 < i) s1 += L[i * LB + k + 1] * x[k + 1];
+                if (k + 2 < i) s2 += L[i * LB + k + 2] * x[k + 2];
+                if (k + 3 < i) s3 += L[i * LB + k + 3] * x[k + 3];
+            }
+            v = -((s0 + s1) + (s2 + s3));
+        }
+        x[i] = v;
+    }
+    if (threadIdx.x < RB) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i) Linv[i * LB + j] = x[i];
+    }
+}
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long k, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    return __longlong_as_double((long long)readlane64((unsigned long long)__double_as_longlong(v), l));
+}
+
+// Inverse of the upper-triangular block D (row stride LB, entries below the
+// diagonal ignored; rows >= jb must be unit rows) into Uinv (row stride LB).
+// Lanes 0..RB-1 of wavefront 0 each back-substitute one column j in
+// registers: x_i = (delta_ij - sum_{k>i} D[i][k] x_k) / D[i][i]; entries
+// below the diagonal come out zero without special cases.  Lane i computes
+// 1/D[i][i] once and the others read it back through readlane.
+__device__ __attribute__((noinline)) void upper_inverse(const double* D, double* Uinv) {
+    if (threadIdx.x >= 64) return;
+    const int lane = (int)threadIdx.x;
+    const int j = lane & (RB - 1);
+    const double dinv = 1.0 / D[j * LB + j];
+    double x[RB];
+#pragma unroll
+    for (int i = RB - 1; i >= 0; --i) {
+        double s0 = (i == j) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int k = i + 1; k < RB; k += 4) {
+            s0 -= D[i * LB + k] * x[k];
+            if (k + 1 < RB) s1 -= D[i * LB + k + 1] * x[k + 1];
+            if (k + 2 < RB) s2 -= D[i * LB + k + 2] * x[k + 2];
+            if (k + 3 < RB) s3 -= D[i * LB + k + 3] * x[k + 3];
+        }
+        x[i] = ((s0 + s1) + (s2 + s3)) * readlane_f64(dinv, i);
+    }
+    if (lane < RB) {
+#pragma unroll
+        for (int i = 0; i < RB; ++i) Uinv[i * LB + j] = x[i];
+    }
+}
+
+// One DPP step of a wavefront max over 64-bit keys.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_max_step(unsigned long long k) {
+    const int lo = (int)(uint32_t)k, hi = (int)(uint32_t)(k >> 32);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    const unsigned long long o = ((unsigned long long)(uint32_t)hi2 << 32) | (uint32_t)lo2;
+    return o > k ? o : k;
+}
+
+// Maximum of a 64-bit key over the wavefront, uniform result: DPP
+// butterflies inside each 16-lane row (quad_perm xor 1, xor 2, half mirror,
+// mirror), then the four row maxima through readlane.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+    k = dpp_max_step<0xB1>(k);
+    k = dpp_max_step<0x4E>(k);
+    k = dpp_max_step<0x141>(k);
+    k = dpp_max_step<0x140>(k);
+    unsigned long long m = readlane64(k, 0);
+    const unsigned long long m1 = readlane64(k, 16), m2 = readlane64(k, 32), m3 = readlane64(k, 48);
+    m = m1 > m ? m1 : m;
+    m = m2 > m ? m2 : m;
+    m = m3 > m ? m3 : m;
+    return m;
+}
+
+// Pivot key of a candidate row: |v| as double bits (monotone for |v| >= 0)
+// with the low 12 mantissa bits replaced by 4095 - row, so the largest key
+// is the largest |v|, candidates within 2^-40 relative counting as equal and
+// the lowest row winning.  Non-candidates have key 0.
+__device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+    return (b & ~0xFFFull) | (unsigned long long)(4095 - row);
+}
+
+template <typename T, int RPT>
+__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_refresh_blocked(RefreshArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int job = (int)blockIdx.x;
+    if (job >= a.njobs) return;
+    const int w = a.jobs[job >> 1];
+    const int spin = job & 1;
+    const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
+    const int nrows = refresh_nrows(ns), ldm = refresh_ldm(nmax);
+    const int nr16 = rup_i(ns, 16);      // logical rows that carry data
+    const int nc16 = rup_i(N, 16);       // columns that carry data
+    const int m16 = rup_i(ns - N, 16);   // empty-site rows, padded
+    const double* __restrict__ UT = spin ? a.UTdn : a.UTup;   // ns x N row-major
+    const int* __restrict__ kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
+    double* __restrict__ M = a.ws + (size_t)job * refresh_ws_job(ns, nmax);
+    double* __restrict__ Lsave = M + (size_t)nrows * ldm;   // [npanel][RB][RB] inverses of L11
+    const int t = (int)threadIdx.x, wave = t / 64, lane = t % 64;
+
+    // ---- LDS carve-up ----------------------------------------------------
+    double* cand = reinterpret_cast<double*>(lds);          // [2][NWAVE][RB]
+    unsigned long long* candk =
+        reinterpret_cast<unsigned long long*>(cand + 2 * NWAVE * RB);   // [2][NWAVE]
+    double* Dblk = reinterpret_cast<double*>(candk + 2 * NWAVE);    // [RB][LB] factored diagonal block
+    double* Linv = Dblk + RB * LB;                          // [RB][LB] U11^-1, then L11^-1
+    const int su = max(nc16 - RB, 16);                      // U12 row stride in LDS
+    double* stage = Linv + RB * LB;                         // U12 [RB][su]
+    const int stage_len = RB * su;
+    double* pivabs = stage + stage_len;                     // [nmax] |pivot|
+    double* redd = pivabs + nmax;                           // [3 * NWAVE]
+    int* wsum = reinterpret_cast<int*>(redd + 3 * NWAVE);   // [NWAVE]
+    int* phys = wsum + NWAVE;                               // [nrows] logical -> physical row
+    int* rowsite = phys + nrows;                            // [ns] physical row -> site
+    int* rowof = rowsite + ns;                              // [ns] site -> physical row
+    int* pinv = rowof + ns;                                 // [nmax] label -> logical row
+    __shared__ int s_bad;
+
+    // ---- rows of M from kappa (ballot prefix sum over empty sites) --------
+    if (t == 0) s_bad = 0;
+    for (int k = t; k < N; k += RT) pinv[k] = 0;
+    for (int r = t; r < nrows; r += RT) phys[r] = r;
+    __syncthreads();
+    int base = 0;
+    for (int s0 = 0; s0 < ns; s0 += RT) {
+        const int s = s0 + t;
+        const int l = s < ns ? kap[s] : -1;
+        const bool empty = (l == 0);
+        const unsigned long long m = __ballot(empty);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int q = 0; q < wave; ++q) off += wsum[q];
+        int tot = 0;
+        for (int q = 0; q < NWAVE; ++q) tot += wsum[q];
+        if (s < ns) {
+            if (empty) {
+                const int row = N + off + before;
+                if (row < ns) { rowsite[row] = s; rowof[s] = row; }
+                else s_bad = 1;
+            } else if (l < 1 || l > N) {
+                s_bad = 1;
+            } else {
+                rowsite[l - 1] = s;
+                rowof[s] = l - 1;
+                if (atomicAdd(&pinv[l - 1], 1) != 0) s_bad = 1;   // pinv doubles as a label count
+            }
+        }
+        base += tot;
+        __syncthreads();
+    }
+    if (t == 0 && base != ns - N) s_bad = 1;
+    __syncthreads();
+    if (s_bad) {
+        if (t == 0) { a.status[w] = 2; atomicMin(a.failed, w); }
+        return;
+    }
+    for (int r0 = 4 * wave; r0 < nrows; r0 += 4 * NWAVE) {
+        for (int c = lane; c < ldm; c += 64) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + u;
+                v[u] = (r < ns && c < N) ? UT[(size_t)rowsite[r] * N + c] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (r0 + u < nrows) M[(size_t)(r0 + u) * ldm + c] = v[u];
+        }
+    }
+    __syncthreads();
+
+    // ---- 1. blocked LU ---------------------------------------------------
+    bool singular = false;
+    for (int j0 = 0; j0 < N && !singular; j0 += RB) {
+        const int jb = min(RB, N - j0);
+        // Registers hold the panel slice of the physical rows myp; myr is the
+        // logical row they currently stand for.
+        double x[RPT][RB];
+        int myr[RPT], myp[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int lr = j0 + t + q * RT;
+            myr[q] = lr;
+            myp[q] = lr < N ? phys[lr] : -1;
+#pragma unroll
+            for (int c = 0; c < RB; ++c)
+                x[q][c] = (myp[q] >= 0) ? M[(size_t)myp[q] * ldm + j0 + c] : 0.0;
+        }
+#pragma clang loop unroll(full)
+        for (int jj = 0; jj < RB; ++jj) {
+            if (jj < jb && !singular) {
+                const int kk = j0 + jj;
+                const int par = jj & 1;
+                unsigned long long key = 0ull;
+                int bq = -1;
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    if (myr[q] >= kk && myr[q] < N) {
+                        const unsigned long long kq = pivot_key(x[q][jj], myr[q]);
+                        if (kq > key) { key = kq; bq = q; }
+                    }
+                }
+                const unsigned long long wkey = wave_max_u64(key);
+                // the lane owning the wavefront's best row publishes it
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    if (bq == q && key == wkey && wkey != 0ull) {
+#pragma unroll
+                        for (int c = 0; c < RB; ++c) cand[(par * NWAVE + wave) * RB + c] = x[q][c];
+                    }
+                }
+                if (lane == 0) candk[par * NWAVE + wave] = wkey;
+                __syncthreads();
+                unsigned long long gk = candk[par * NWAVE];
+                int gw = 0;
+                for (int q = 1; q < NWAVE; ++q) {
+                    const unsigned long long k2 = candk[par * NWAVE + q];
+                    if (k2 > gk) { gk = k2; gw = q; }
+                }
+                const int p = gk ? 4095 - (int)(gk & 0xFFFull) : 0x7fffffff;
+                // lane c holds pivot-row entry c; the update reads them back
+                // through readlane as scalar operands (no per-entry LDS trips)
+                const double pv = (p < N) ? cand[(par * NWAVE + gw) * RB + (lane & (RB - 1))] : 0.0;
+                const double piv = readlane_f64(pv, jj);
+                if (!(fabs(piv) > 0.0) || !isfinite(piv)) {
+                    singular = true;
+                } else {
+                    if (t == 0) {
+                        const int tmp = phys[kk]; phys[kk] = phys[p]; phys[p] = tmp;
+                        pivabs[kk] = fabs(piv);
+                    }
+                    const double rp = 1.0 / piv;
+#pragma unroll
+                    for (int q = 0; q < RPT; ++q) {
+                        const int r = myr[q];
+                        myr[q] = (r == p) ? kk : (r == kk ? p : r);
+                        if (myr[q] > kk && myp[q] >= 0) {
+                            const double l = x[q][jj] * rp;
+                            x[q][jj] = l;
+#pragma unroll
+                            for (int c = jj + 1; c < RB; ++c) x[q][c] -= l * readlane_f64(pv, c);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (singular) break;
+        // panel back to its physical rows; the factored diagonal block to LDS
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (myp[q] >= 0) {
+#pragma unroll
+                for (int c = 0; c < RB; ++c) M[(size_t)myp[q] * ldm + j0 + c] = x[q][c];
+            }
+            const int i = myr[q] - j0;
+            if (myp[q] >= 0 && i >= 0 && i < RB) {
+#pragma unroll
+                for (int c = 0; c < RB; ++c) Dblk[i * LB + c] = x[q][c];
+            }
+        }
+        for (int i = jb + t; i < RB; i += RT) {
+            for (int c = 0; c < RB; ++c) Dblk[i * LB + c] = (c == i) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        // empty-site rows: their panel slice solves x U11 = a  ->  x = a U11^-1
+        upper_inverse(Dblk, Linv);
+        __syncthreads();
+        {
+            const int ntr = m16 / 16;
+            for (int ti = wave; ti < ntr; ti += NWAVE) {
+                const int ra = N + 16 * ti + (lane & 15);
+                f64x4 y0 = {0, 0, 0, 0}, y1 = {0, 0, 0, 0};
+#pragma unroll
+                for (int s2 = 0; s2 < RB / 4; ++s2) {
+                    const int k = 4 * s2 + (lane >> 4);
+                    const double av = M[(size_t)ra * ldm + j0 + k];
+                    y0 = mfma16(av, Linv[k * LB + (lane & 15)], y0);
+                    y1 = mfma16(av, Linv[k * LB + 16 + (lane & 15)], y1);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const size_t row = (size_t)(N + 16 * ti + (lane >> 4) + 4 * g) * ldm + j0;
+                    M[row + (lane & 15)] = y0[g];
+                    M[row + 16 + (lane & 15)] = y1[g];
+                }
+            }
+        }
+        __syncthreads();
+        for (int idx = t; idx < RB * RB; idx += RT) {
+            const int i = idx / RB, c = idx % RB;
+            if (c >= i) Dblk[i * LB + c] = 0.0;      // keep the strictly lower L11
+        }
+        __syncthreads();
+        unit_lower_inverse(Dblk, Linv, jb);
+        __syncthreads();
+        for (int idx = t; idx < RB * RB; idx += RT)
+            Lsave[(size_t)(j0 / RB) * RB * RB + idx] = Linv[(idx / RB) * LB + idx % RB];
+        const int c0 = j0 + RB;
+        if (c0 >= nc16) continue;
+        // U12 = L11^-1 A12 on MFMA: one 16-column block per wavefront
+        const int ntc = (nc16 - c0) / 16;
+        for (int tj = wave; tj < ntc; tj += NWAVE) {
+            const int cc = 16 * tj + (lane & 15);
+            f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < RB / 4; ++s) {
+                const int k = 4 * s + (lane >> 4);
+                const double b = M[(size_t)phys[j0 + k] * ldm + c0 + cc];
+                u0 = mfma16(Linv[(lane & 15) * LB + k], b, u0);
+                u1 = mfma16(Linv[(16 + (lane & 15)) * LB + k], b, u1);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int r0 = (lane >> 4) + 4 * g, r1 = r0 + 16;
+                M[(size_t)phys[j0 + r0] * ldm + c0 + cc] = u0[g];
+                M[(size_t)phys[j0 + r1] * ldm + c0 + cc] = u1[g];
+                stage[r0 * su + cc] = u0[g];
+                stage[r1 * su + cc] = u1[g];
+            }
+        }
+        __syncthreads();
+        // trailing update: M[r][c] -= L21[r][:] U12[:][c]
+        const int r0 = c0;
+        const int ntr = (nr16 - r0) / 16;
+        for (int ti = wave; ti < ntr; ti += NWAVE) {
+            const int pa = phys[r0 + 16 * ti + (lane & 15)];
+            double af[RB / 4];
+#pragma unroll
+            for (int s = 0; s < RB / 4; ++s) af[s] = -M[(size_t)pa * ldm + j0 + 4 * s + (lane >> 4)];
+            double* crow[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                crow[g] = M + (size_t)phys[r0 + 16 * ti + (lane >> 4) + 4 * g] * ldm + c0 + (lane & 15);
+            f64x4 acc, nxt;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) acc[g] = crow[g][0];
+            for (int tj = 0; tj < ntc; ++tj) {
+                const int cc = 16 * tj + (lane & 15);
+                if (tj + 1 < ntc) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) nxt[g] = crow[g][16 * (tj + 1)];
+                }
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s)
+                    acc = mfma16(af[s], stage[(4 * s + (lane >> 4)) * su + cc], acc);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) crow[g][16 * tj] = acc[g];
+                acc = nxt;
+            }
+        }
+        __syncthreads();
+    }
+    double logdet = 0.0, pmin = INFINITY, pmax = 0.0;
+    if (!singular) {
+        double ls = 0.0, mn = INFINITY, mx = 0.0;
+        for (int k = t; k < N; k += RT) {
+            const double v = pivabs[k];
+            ls += log(v);
+            mn = fmin(mn, v);
+            mx = fmax(mx, v);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ls += __shfl_xor(ls, o, 64);
+            mn = fmin(mn, __shfl_xor(mn, o, 64));
+            mx = fmax(mx, __shfl_xor(mx, o, 64));
+        }
+        if (lane == 0) { redd[wave] = ls; redd[NWAVE + wave] = mn; redd[2 * NWAVE + wave] = mx; }
+        __syncthreads();
+        for (int q = 0; q < NWAVE; ++q) {
+            logdet += redd[q];
+            pmin = fmin(pmin, redd[NWAVE + q]);
+            pmax = fmax(pmax, redd[2 * NWAVE + q]);
+        }
+    }
+    if (!singular && pmin < a.pivot_tol * pmax) singular = true;
+    if (singular) {
+        if (t == 0) { a.status[w] = 1; atomicMin(a.failed, w); }
+        return;
+    }
+    if (t == 0) {
+        a.logdet[(size_t)w * 2 + spin] = logdet;
+        if (a.write_w && spin == 0) { a.npend[w] = 0; a.acc_since[w] = 0; }
+    }
+    // logical row c < N holds label phys[c]
+    for (int c = t; c < N; c += RT) pinv[phys[c]] = c;
+
+    // ---- 2. X L1 = L2 for the empty-site rows (physical = logical rows >= N)
+    const int npanel = (N + RB - 1) / RB;
+    for (int pq = npanel - 1; pq >= 0; --pq) {
+        const int j0 = pq * RB, jb = min(RB, N - j0);
+        for (int idx = t; idx < RB * RB; idx += RT)
+            Linv[(idx / RB) * LB + idx % RB] = Lsave[(size_t)pq * RB * RB + idx];
+        __syncthreads();
+        // X_J = B_J L1_JJ^-1, one 16-row tile per wavefront (both column tiles)
+        const int ntr = m16 / 16;
+        for (int ti = wave; ti < ntr; ti += NWAVE) {
+            const int ra = N + 16 * ti + (lane & 15);
+            f64x4 x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < RB / 4; ++s) {
+                const int k = 4 * s + (lane >> 4);
+                const double av = M[(size_t)ra * ldm + j0 + k];
+                x0 = mfma16(av, Linv[k * LB + (lane & 15)], x0);
+                x1 = mfma16(av, Linv[k * LB + 16 + (lane & 15)], x1);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int rr = 16 * ti + (lane >> 4) + 4 * g;
+                M[(size_t)(N + rr) * ldm + j0 + (lane & 15)] = x0[g];
+                M[(size_t)(N + rr) * ldm + j0 + 16 + (lane & 15)] = x1[g];
+            }
+        }
+        __syncthreads();
+        // M[N.., 0:j0] -= X_J L1[J, 0:j0]
+        const int ntc = j0 / 16;
+        for (int ti = wave; ti < ntr; ti += NWAVE) {
+            double af[RB / 4];
+#pragma unroll
+            for (int s = 0; s < RB / 4; ++s)
+                af[s] = -M[(size_t)(N + 16 * ti + (lane & 15)) * ldm + j0 + 4 * s + (lane >> 4)];
+            int pb[RB / 4];
+#pragma unroll
+            for (int s = 0; s < RB / 4; ++s) {
+                const int k = 4 * s + (lane >> 4);
+                pb[s] = k < jb ? phys[j0 + k] : -1;
+            }
+            const int rc = N + 16 * ti + (lane >> 4);
+            double* crow = M + (size_t)rc * ldm + (lane & 15);
+            f64x4 acc, nxt;
+            double bf[RB / 4], bn[RB / 4];
+            if (ntc > 0) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) acc[g] = crow[(size_t)4 * g * ldm];
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) bf[s] = pb[s] >= 0 ? M[(size_t)pb[s] * ldm + (lane & 15)] : 0.0;
+            }
+            for (int tj = 0; tj < ntc; ++tj) {
+                if (tj + 1 < ntc) {
+                    const int cn = 16 * (tj + 1);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) nxt[g] = crow[(size_t)4 * g * ldm + cn];
+#pragma unroll
+                    for (int s = 0; s < RB / 4; ++s)
+                        bn[s] = pb[s] >= 0 ? M[(size_t)pb[s] * ldm + cn + (lane & 15)] : 0.0;
+                }
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) acc = mfma16(af[s], bf[s], acc);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) crow[(size_t)4 * g * ldm + 16 * tj] = acc[g];
+                acc = nxt;
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) bf[s] = bn[s];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. W from X, label column by label column ------------------------
+    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
+    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+    // wavefronts take label columns, lanes take sites; each lane keeps up to
+    // 8 sites' kappa and row in registers per 512-site chunk.
+    const int nsp = rup_i(ns, 4);
+    const int cpend = a.write_w ? 0 : a.npend[w];
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
+    double dmax = 0.0, vmax = 0.0;
+    for (int sb = 0; sb < ns; sb += 512) {
+        int ks[8], ro[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int s = sb + lane + 64 * q;
+            ks[q] = s < ns ? kap[s] : -1;
+            ro[q] = (s < ns && ks[q] == 0) ? rowof[s] : 0;
+        }
+        for (int l = wave; l < N; l += NWAVE) {
+            const int pl = pinv[l];
+            T* __restrict__ wcol = Wo + (size_t)l * ns;
+            double vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                vv[q] = (ks[q] > 0) ? ((ks[q] - 1 == l) ? 1.0 : 0.0)
+                                    : (ks[q] == 0 ? M[(size_t)ro[q] * ldm + pl] : 0.0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int s = sb + lane + 64 * q;
+                if (ks[q] < 0) continue;
+                const double v = vv[q];
+                if (a.write_w) {
+                    wcol[s] = (T)v;
+                } else {
+                    double cur = (double)wcol[s];
+                    for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + s] * (double)gB[c * nmax + l];
+                    dmax = fmax(dmax, fabs(v - cur));
+                    vmax = fmax(vmax, fabs(v));
+                }
+            }
+        }
+    }
+    if (!a.write_w) {
+        atomic_max_nonneg2(&a.drift[0], dmax);
+        atomic_max_nonneg2(&a.drift[1], vmax);
+    }
+}
+
+size_t refresh_ws_doubles_per_job(int ns, int nmax) { return refresh_ws_job(ns, nmax); }
+
+static size_t refresh_lds_bytes(int ns, int nup, int ndn) {
+    const int nmax = nup > ndn ? nup : ndn;
+    const int nmin = nup < ndn ? nup : ndn;
+    const int su = rup_i(nmax, 16) - RB > 16 ? rup_i(nmax, 16) - RB : 16;
+    const int stage = RB * su;
+    (void)nmin;
+    const size_t dbl = (size_t)(2 * NWAVE * RB + 2 * NWAVE + 2 * RB * LB + stage + nmax + 3 * NWAVE) * sizeof(double);
+    const size_t ints = (size_t)(NWAVE + refresh_nrows(ns) + 2 * ns + nmax) * sizeof(int);
+    return dbl + ints + 16;
+}
+
+static int refresh_rows_per_thread(int nmax) {
+    return (nmax + RT - 1) / RT;
+}
+
+template <int RPT>
+static hipError_t launch_refresh_r(int prec, const RefreshArgs& a, size_t lds, hipStream_t s) {
+    const void* fn = prec == 0 ? reinterpret_cast<const void*>(&k_refresh_blocked<double, RPT>)
+                               : reinterpret_cast<const void*>(&k_refresh_blocked<float, RPT>);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (prec == 0) hipLaunchKernelGGL((k_refresh_blocked<double, RPT>), dim3(a.njobs), dim3(RT), lds, s, a);
+    else hipLaunchKernelGGL((k_refresh_blocked<float, RPT>), dim3(a.njobs), dim3(RT), lds, s, a);
+    return hipGetLastError();
+}
+
+bool refresh_supported(int ns, int nup, int ndn) {
+    return refresh_lds_bytes(ns, nup, ndn) <= 160 * 1024 && refresh_rows_per_thread(nup > ndn ? nup : ndn) <= 4;
+}
+
+hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s) {
+    const size_t lds = refresh_lds_bytes(a.ns, a.nup, a.ndn);
+    if (!refresh_supported(a.ns, a.nup, a.ndn)) return hipErrorInvalidValue;
+    switch (refresh_rows_per_thread(a.nmax)) {
+        case 1: return launch_refresh_r<1>(prec, a, lds, s);
+        case 2: return launch_refresh_r<2>(prec, a, lds, s);
+        default: return launch_refresh_r<4>(prec, a, lds, s);
+    }
+}
+
+}  // namespace kvmc

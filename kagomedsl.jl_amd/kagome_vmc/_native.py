@@ -181,7 +181,7 @@ class DeviceSampler:
         _check(load().kvmc_get_configs(self._h, _ptr(ku), _ptr(kd)))
         return ku, kd
 
-    def init_random(self, seed, max_attempts=16):
+    def init_random(self, seed, max_attempts=64):
         n = C.c_int64(0)
         _check(load().kvmc_init_random(self._h, seed, max_attempts, C.byref(n)))
         return n.value

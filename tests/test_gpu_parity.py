@@ -23,11 +23,11 @@ def _max_rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("L", [2, 4, 8])
+@pytest.mark.parametrize("L", [2, 4, 6, 8, 12])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_refresh_matches_oracle(oracle_mod, L, prec):
     ham = system(L)
-    nw = 6
+    nw = 6 if L < 12 else 2
     ku, kd = well_conditioned_configs(oracle_mod, 11, nw, ham)
     with sampler(ham, nw, precision=prec) as ds:
         ds.set_configs(ku, kd)
@@ -39,6 +39,38 @@ def test_refresh_matches_oracle(oracle_mod, L, prec):
             assert _max_rel(ds.get_w(w, 0), Wu_o.real) < tol
             assert _max_rel(ds.get_w(w, 1), Wd_o.real) < tol
             assert abs(logdet[w] - (lu + ld)) < 1e-9 * max(1.0, abs(lu + ld))
+
+
+@pytest.mark.parametrize("n_up", [25, 30])
+def test_refresh_unequal_filling(oracle_mod, n_up):
+    """N_up != N_down (the reference's imbalance scan, scripts/FP.jl:112-116)."""
+    from kagome_vmc import DoubleKagome, Hamiltonian
+    lat = DoubleKagome.build(1.0, 4, 4, (True, True), (False, True))
+    ham = Hamiltonian.build(n_up, 48 - n_up, lat)
+    nw = 4
+    ku = np.zeros((nw, 48), dtype=np.int32)
+    kd = np.zeros((nw, 48), dtype=np.int32)
+    w = q = 0
+    while w < nw:
+        a, b = oracle_mod.random_config(3, q, 0, 48, n_up, 48 - n_up)
+        q += 1
+        if max(np.linalg.cond(oracle_mod.tilde_u(ham.U_up, a)),
+               np.linalg.cond(oracle_mod.tilde_u(ham.U_down, b))) < 1e6:
+            ku[w], kd[w] = a, b
+            w += 1
+    with sampler(ham, nw) as ds:
+        ds.set_configs(ku, kd)
+        for w in range(nw):
+            assert _max_rel(ds.get_w(w, 0), oracle_mod.refresh_w(ham.U_up, ku[w]).real) < W_RTOL
+            assert _max_rel(ds.get_w(w, 1), oracle_mod.refresh_w(ham.U_down, kd[w]).real) < W_RTOL
+        ds.sweep(50)
+        e, _ = ds.measure()
+        ku2, kd2 = ds.get_configs()
+        for w in range(nw):
+            Wu = oracle_mod.refresh_w(ham.U_up, ku2[w])
+            Wd = oracle_mod.refresh_w(ham.U_down, kd2[w])
+            eo = oracle_mod.getol(ham.nn, ku2[w], kd2[w], Wu, Wd)
+            assert abs(e[w] - eo) < 1e-8 * max(1.0, abs(eo))
 
 
 @pytest.mark.parametrize("L", [4, 8])
@@ -69,7 +101,7 @@ def test_init_random_matches_oracle_draws(oracle_mod):
         ku, kd = ds.get_configs()
     assert redraws >= 0
     for w in range(nw):
-        for attempt in range(17):
+        for attempt in range(65):
             a, b = oracle_mod.random_config(99, w, attempt, ham.ns, ham.N_up, ham.N_down)
             if np.array_equal(a, ku[w]) and np.array_equal(b, kd[w]):
                 break
