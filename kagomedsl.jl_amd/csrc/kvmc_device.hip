@@ -46,6 +46,7 @@ __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __res
                                               const T* __restrict__ sB, int ns, int nsp,
                                               int nmax, int ncols, int c) {
     using VT = typename VecT<T, V>::type;
+    constexpr int UNR = 4;   // columns in flight per thread
     const int nvec = ns / V;
     const int lanes = (int)blockDim.x / nvec;
     const int t = (int)threadIdx.x;
@@ -54,24 +55,26 @@ __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __res
     VT a[DMAX];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q)
-        if (q < c) a[q] = *reinterpret_cast<const VT*>(sA + (size_t)q * nsp + (size_t)iv * V);
-    T* base = W0 + (size_t)iv * V;
+        a[q] = (q < c) ? *reinterpret_cast<const VT*>(sA + (size_t)q * nsp + (size_t)iv * V) : VT(0);
+    T* __restrict__ base = W0 + (size_t)iv * V;
     int j = lane;
-    for (; j + lanes < ncols; j += 2 * lanes) {
-        VT w0 = *reinterpret_cast<VT*>(base + (size_t)j * ns);
-        VT w1 = *reinterpret_cast<VT*>(base + (size_t)(j + lanes) * ns);
+    for (; j + (UNR - 1) * lanes < ncols; j += UNR * lanes) {
+        VT w[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) w[u] = *reinterpret_cast<const VT*>(base + (size_t)(j + u * lanes) * ns);
 #pragma unroll
         for (int q = 0; q < DMAX; ++q) {
             if (q < c) {
-                w0 += a[q] * sB[q * nmax + j];
-                w1 += a[q] * sB[q * nmax + j + lanes];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) w[u] += a[q] * sB[q * nmax + j + u * lanes];
             }
         }
-        *reinterpret_cast<VT*>(base + (size_t)j * ns) = w0;
-        *reinterpret_cast<VT*>(base + (size_t)(j + lanes) * ns) = w1;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) *reinterpret_cast<VT*>(base + (size_t)(j + u * lanes) * ns) = w[u];
     }
-    if (j < ncols) {
-        VT w0 = *reinterpret_cast<VT*>(base + (size_t)j * ns);
+#pragma nounroll
+    for (; j < ncols; j += lanes) {
+        VT w0 = *reinterpret_cast<const VT*>(base + (size_t)j * ns);
 #pragma unroll
         for (int q = 0; q < DMAX; ++q)
             if (q < c) w0 += a[q] * sB[q * nmax + j];

@@ -181,8 +181,8 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     double* Dblk = reinterpret_cast<double*>(candk + 2 * NWAVE);    // [RB][LB] factored diagonal block
     double* Linv = Dblk + RB * LB;                          // [RB][LB] U11^-1, then L11^-1
     const int su = max(nc16 - RB, 16);                      // U12 row stride in LDS
-    double* stage = Linv + RB * LB;                         // U12 [RB][su]
-    const int stage_len = RB * su;
+    double* stage = Linv + RB * LB;                         // U12 [RB][su] / scatter block
+    const int stage_len = max(RB * su, 16 * (nc16 | 1));
     double* pivabs = stage + stage_len;                     // [nmax] |pivot|
     double* redd = pivabs + nmax;                           // [3 * NWAVE]
     int* wsum = reinterpret_cast<int*>(redd + 3 * NWAVE);   // [NWAVE]
@@ -537,44 +537,43 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     // ---- 3. W from X, label column by label column ------------------------
     const size_t wstride = (size_t)ns * (a.nup + a.ndn);
     T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
-    // wavefronts take label columns, lanes take sites; each lane keeps up to
-    // 8 sites' kappa and row in registers per 512-site chunk.
+    // Blocks of SB consecutive sites: the solved rows of the block's empty
+    // sites are staged in LDS (contiguous row reads, odd row stride against
+    // bank conflicts), then every label column of W gets its SB entries of the
+    // block written as one contiguous run.
+    constexpr int SB = 16;
+    const int xs = (nc16 | 1);                       // LDS row stride (odd)
+    double* xb = stage;                              // [SB][xs]
     const int nsp = rup_i(ns, 4);
     const int cpend = a.write_w ? 0 : a.npend[w];
     const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
     double dmax = 0.0, vmax = 0.0;
-    for (int sb = 0; sb < ns; sb += 512) {
-        int ks[8], ro[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int s = sb + lane + 64 * q;
-            ks[q] = s < ns ? kap[s] : -1;
-            ro[q] = (s < ns && ks[q] == 0) ? rowof[s] : 0;
-        }
-        for (int l = wave; l < N; l += NWAVE) {
-            const int pl = pinv[l];
-            T* __restrict__ wcol = Wo + (size_t)l * ns;
-            double vv[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                vv[q] = (ks[q] > 0) ? ((ks[q] - 1 == l) ? 1.0 : 0.0)
-                                    : (ks[q] == 0 ? M[(size_t)ro[q] * ldm + pl] : 0.0);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int s = sb + lane + 64 * q;
-                if (ks[q] < 0) continue;
-                const double v = vv[q];
-                if (a.write_w) {
-                    wcol[s] = (T)v;
-                } else {
-                    double cur = (double)wcol[s];
-                    for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + s] * (double)gB[c * nmax + l];
-                    dmax = fmax(dmax, fabs(v - cur));
-                    vmax = fmax(vmax, fabs(v));
-                }
+    for (int sb = 0; sb < ns; sb += SB) {
+        for (int i = wave; i < SB; i += NWAVE) {
+            const int s = sb + i;
+            if (s < ns && kap[s] == 0) {
+                const double* __restrict__ src = M + (size_t)rowof[s] * ldm;
+                for (int c = lane; c < N; c += 64) xb[i * xs + c] = src[c];
             }
         }
+        __syncthreads();
+        for (int idx = t; idx < N * SB; idx += RT) {
+            const int l = idx / SB, i = idx % SB, s = sb + i;
+            if (s >= ns) continue;
+            const int ks = kap[s];
+            const double v = ks != 0 ? ((ks - 1 == l) ? 1.0 : 0.0) : xb[i * xs + pinv[l]];
+            T* __restrict__ dst = Wo + (size_t)l * ns + s;
+            if (a.write_w) {
+                *dst = (T)v;
+            } else {
+                double cur = (double)*dst;
+                for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + s] * (double)gB[c * nmax + l];
+                dmax = fmax(dmax, fabs(v - cur));
+                vmax = fmax(vmax, fabs(v));
+            }
+        }
+        __syncthreads();
     }
     if (!a.write_w) {
         atomic_max_nonneg2(&a.drift[0], dmax);
@@ -588,7 +587,7 @@ static size_t refresh_lds_bytes(int ns, int nup, int ndn) {
     const int nmax = nup > ndn ? nup : ndn;
     const int nmin = nup < ndn ? nup : ndn;
     const int su = rup_i(nmax, 16) - RB > 16 ? rup_i(nmax, 16) - RB : 16;
-    const int stage = RB * su;
+    const int stage = RB * su > 16 * (rup_i(nmax, 16) | 1) ? RB * su : 16 * (rup_i(nmax, 16) | 1);
     (void)nmin;
     const size_t dbl = (size_t)(2 * NWAVE * RB + 2 * NWAVE + 2 * RB * LB + stage + nmax + 3 * NWAVE) * sizeof(double);
     const size_t ints = (size_t)(NWAVE + refresh_nrows(ns) + 2 * ns + nmax) * sizeof(int);
