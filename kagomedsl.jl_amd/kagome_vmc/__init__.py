@@ -11,11 +11,14 @@ from .lattice import (DoubleKagome, get_boundary_shifts, get_site_coord, ns, uni
 from .hamiltonian import Hamiltonian, Hmat, get_nn, orbitals, pi_link_in, pi_link_inter, shell_gap
 from .montecarlo import (MC, MCContext, init, measure, read_checkpoint, register_evaluables, run,
                          sweep, write_checkpoint)
+from .gradient import (AnsatzFamily, GradientSampler, ansatz_family, energy, exhaustive_average,
+                       orbital_derivatives)
 from . import _native as native
 
 __all__ = [
     "DoubleKagome", "ns", "unitcell_coord", "unitcell_diff", "get_site_coord",
     "get_boundary_shifts", "Hamiltonian", "Hmat", "get_nn", "orbitals", "pi_link_in",
     "pi_link_inter", "shell_gap", "MC", "MCContext", "init", "sweep", "measure", "run",
-    "register_evaluables", "write_checkpoint", "read_checkpoint", "native",
+    "register_evaluables", "write_checkpoint", "read_checkpoint", "native", "AnsatzFamily",
+    "GradientSampler", "ansatz_family", "energy", "exhaustive_average", "orbital_derivatives",
 ]

@@ -450,3 +450,38 @@ def exhaustive(bonds, Uup, Udn):
         "eloc": E, "weight": w, "zmu": Zmu, "logdet": np.array(logdet), "confs": confs,
         "psi_spin": psi,
     }
+
+
+# --------------------------------------------------------------------------
+# Differentiation (no counterpart in the reference, which has none)
+# --------------------------------------------------------------------------
+def log_derivs(U, dU_list, kappa):
+    """O_k = d log det tildeU / d theta_k = tr(tildeU^-1 d tildeU_k), with
+    tildeU = U[R, :] and d tildeU_k = dU_k[R, :] (rows in label order), by a
+    dense solve per parameter.  Pinned against finite differences of
+    log|det tildeU(theta)|."""
+    Ut = tilde_u(U, kappa)
+    out = np.zeros(len(dU_list))
+    for k, dU in enumerate(dU_list):
+        dUt = tilde_u(np.asarray(dU, dtype=np.complex128), kappa)
+        out[k] = float(np.real(np.trace(np.linalg.solve(Ut, dUt))))
+    return out
+
+
+def exhaustive_grad(bonds, Uup, Udn, dUup, dUdn):
+    """|psi|^2-weighted <E>, <O_k>, <E_L O_k> and S_kl = <O_k O_l> - <O_k><O_l>
+    over every configuration, plus grad = 2(<E_L O_k> - <E><O_k>)."""
+    r = exhaustive(bonds, Uup, Udn)
+    ns, nup = Uup.shape
+    P = len(dUup)
+    O = np.zeros((len(r["eloc"]), P))
+    for q, (kup, kdn) in enumerate(all_configs(ns, nup)):
+        if r["weight"][q] > 0:
+            O[q] = log_derivs(Uup, dUup, kup) + log_derivs(Udn, dUdn, kdn)
+    w = r["weight"] / r["weight"].sum()
+    E = float(w @ r["eloc"])
+    Om = w @ O
+    EO = (w * r["eloc"]) @ O
+    S = (O * w[:, None]).T @ O - np.outer(Om, Om)
+    r.update({"O_per_config": O, "O": Om, "EO": EO, "S": S, "grad": 2.0 * (EO - E * Om)})
+    return r

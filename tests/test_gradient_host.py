@@ -1,0 +1,189 @@
+"""Variational-gradient host math, pinned without the GPU.
+
+The reference has no differentiation, so nothing here has a reference golden
+vector; each piece is pinned by an independent route instead:
+  * orbital derivatives   -- finite differences of the occupied projector,
+  * oracle.log_derivs     -- finite differences of log|det tildeU(theta)|,
+  * the energy gradient   -- finite differences of the exact 12-site energy
+                             psi^T H psi / psi^T psi (dense Heisenberg matrix),
+  * the S matrix          -- the fidelity metric 1 - |<psi(a)|psi(b)>|^2,
+  * the autograd wrapper  -- torch.autograd.gradcheck.
+"""
+import numpy as np
+import pytest
+import torch
+
+from kagome_vmc import DoubleKagome, Hmat, get_nn
+from kagome_vmc.lattice import site_coords
+from kagome_vmc.gradient import (NPARAMS, THETA0, _torus_distance, ansatz_family, averages,
+                                 energy, orbital_derivatives)
+from kagome_systems import BC
+
+THETA1 = np.array([0.93, 0.07, -0.05, 0.11])
+
+
+def family(L):
+    pbc, apbc = BC[L]
+    return ansatz_family(DoubleKagome.build(1.0, L, L, pbc, apbc))
+
+
+def occupied(H, n):
+    w, v = np.linalg.eigh(H)
+    return v[:, :n]
+
+
+@pytest.mark.parametrize("L", [2, 4, 6])
+def test_family_reduces_to_pi_flux(L):
+    f = family(L)
+    lat = f.lat
+    H = f.hamiltonian(THETA0)
+    assert np.array_equal(H, Hmat(lat).real)
+    for G in f.G:
+        assert np.array_equal(G, G.T)
+    # J1 list is the pi-flux one whatever theta is
+    assert f.nn == get_nn(Hmat(lat).real)
+
+
+@pytest.mark.parametrize("L", [4, 6])
+def test_second_neighbour_and_anisotropy_terms(L):
+    f = family(L)
+    ns = f.H_up.shape[0]
+    H = f.hamiltonian(THETA0)
+    G2, Ga1 = f.G[2], f.G[3]
+    # four t2 partners per site, none of them a J1 neighbour, all at sqrt(3)
+    assert np.all(np.count_nonzero(G2, axis=1) == 4)
+    assert not np.any((G2 != 0) & (H != 0))
+    d = _torus_distance(f.lat, site_coords(f.lat))
+    assert np.allclose(d[G2 != 0], np.sqrt(3.0))
+    # a1 term: one third of the J1 bonds, copied with their pi-flux sign
+    assert np.count_nonzero(np.triu(Ga1)) * 3 == len(f.nn)
+    assert np.array_equal(Ga1[Ga1 != 0], H[Ga1 != 0])
+    assert ns == 3 * L * L
+
+
+@pytest.mark.parametrize("L", [2, 4])
+@pytest.mark.parametrize("theta", [THETA0, THETA1])
+def test_orbital_derivative_matches_projector_fd(L, theta):
+    f = family(L)
+    n = 3 * L * L // 2
+    U, dU, _ = orbital_derivatives(f.hamiltonian(theta), f.G, n)
+    h = 1e-5
+    for k in range(NPARAMS):
+        e = np.zeros(NPARAMS); e[k] = h
+        Pp = occupied(f.hamiltonian(theta + e), n)
+        Pm = occupied(f.hamiltonian(theta - e), n)
+        dP_fd = (Pp @ Pp.T - Pm @ Pm.T) / (2 * h)
+        dP = dU[k] @ U.T + U @ dU[k].T
+        assert np.abs(dP - dP_fd).max() < 1e-7, k
+        assert np.abs(U.T @ dU[k]).max() < 1e-12       # parallel-transport gauge
+
+
+def test_open_shell_is_rejected():
+    f = family(2)
+    H = f.hamiltonian(THETA0)
+    w = np.linalg.eigvalsh(H)
+    # find a filling whose Fermi level is degenerate
+    n = next(m for m in range(1, len(w)) if abs(w[m] - w[m - 1]) < 1e-10)
+    with pytest.raises(ValueError, match="open shell"):
+        orbital_derivatives(H, f.G, n)
+
+
+@pytest.mark.parametrize("L", [2, 4])
+def test_oracle_log_derivs_match_fd(oracle_mod, L):
+    f = family(L)
+    n = 3 * L * L // 2
+    ns = 3 * L * L
+    theta = THETA1
+    U, dU, _ = orbital_derivatives(f.hamiltonian(theta), f.G, n)
+    h = 1e-5
+    checked = 0
+    for trial in range(40):
+        ku, _ = oracle_mod.random_config(11, trial, 0, ns, n, ns - n)
+        Ut = oracle_mod.tilde_u(U, ku).real
+        if np.linalg.cond(Ut) > 1e5:
+            continue
+        O = oracle_mod.log_derivs(U, dU, ku)
+        for k in range(NPARAMS):
+            e = np.zeros(NPARAMS); e[k] = h
+            lp = np.linalg.slogdet(oracle_mod.tilde_u(occupied(f.hamiltonian(theta + e), n), ku).real)[1]
+            lm = np.linalg.slogdet(oracle_mod.tilde_u(occupied(f.hamiltonian(theta - e), n), ku).real)[1]
+            assert abs(O[k] - (lp - lm) / (2 * h)) < 1e-5 * max(1.0, abs(O[k])), (trial, k)
+        checked += 1
+        if checked == 6:
+            break
+    assert checked == 6
+
+
+def exact_energy(oracle_mod, f, theta, n):
+    H = f.hamiltonian(theta)
+    U = occupied(H, n)
+    return oracle_mod.exhaustive(f.nn, U, U)["E_quadratic_form"]
+
+
+@pytest.fixture(scope="module")
+def twelve_site(oracle_mod):
+    f = family(2)
+    out = {}
+    for name, th in (("theta0", THETA0), ("theta1", THETA1)):
+        U, dU, _ = orbital_derivatives(f.hamiltonian(th), f.G, 6)
+        out[name] = (th, U, dU, oracle_mod.exhaustive_grad(f.nn, U, U, dU, dU))
+    return f, out
+
+
+@pytest.mark.parametrize("which", ["theta0", "theta1"])
+def test_exhaustive_gradient_matches_energy_fd(oracle_mod, twelve_site, which):
+    f, res = twelve_site
+    theta, U, dU, r = res[which]
+    assert abs(r["E"] - r["E_quadratic_form"]) < 1e-10
+    h = 1e-4
+    for k in range(NPARAMS):
+        e = np.zeros(NPARAMS); e[k] = h
+        fd = (exact_energy(oracle_mod, f, theta + e, 6) - exact_energy(oracle_mod, f, theta - e, 6)) / (2 * h)
+        assert abs(r["grad"][k] - fd) < 1e-6, (k, r["grad"][k], fd)
+
+
+@pytest.mark.parametrize("which", ["theta0", "theta1"])
+def test_s_matrix_is_fidelity_metric(oracle_mod, twelve_site, which):
+    f, res = twelve_site
+    theta, U, dU, r = res[which]
+    confs = r["confs"]
+
+    def psi(th):
+        Uo = occupied(f.hamiltonian(th), 6)
+        p = oracle_mod.spin_amplitudes(Uo, Uo, confs).real
+        return p / np.linalg.norm(p)
+
+    h = 1e-3
+    S = r["S"]
+    for k in range(NPARAMS):
+        for l in range(k, NPARAMS):
+            v = np.zeros(NPARAMS); v[k] += 1; v[l] += 1
+            F = float(psi(theta + 0.5 * h * v) @ psi(theta - 0.5 * h * v)) ** 2
+            assert abs((1.0 - F) / h**2 - v @ S @ v) < 1e-5 * max(1.0, v @ S @ v), (k, l)
+    assert np.all(np.linalg.eigvalsh(S) > -1e-12)
+
+
+def test_averages_weighted_closed_form():
+    eloc = np.array([1.0, 3.0])
+    O = np.array([[2.0], [0.0]])
+    r = averages(eloc, O, weights=[3.0, 1.0])
+    # <E> = 1.5, <O> = 1.5, <EO> = 1.5, S = <O^2> - <O>^2 = 3 - 2.25
+    assert r["E"] == 1.5 and r["O"][0] == 1.5 and r["EO"][0] == 1.5
+    assert r["S"][0, 0] == 0.75 and r["grad"][0] == 2 * (1.5 - 1.5 * 1.5)
+
+
+def test_energy_function_gradcheck_cpu(oracle_mod):
+    """The autograd wrapper's backward against torch's own finite differences
+    at 12 sites, with the oracle's exhaustive averages as the evaluator."""
+    f = family(2)
+
+    def evaluate(th):
+        U, dU, _ = orbital_derivatives(f.hamiltonian(th), f.G, 6)
+        return oracle_mod.exhaustive_grad(f.nn, U, U, dU, dU)
+
+    theta = torch.tensor(THETA1, dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda t: energy(t, evaluate), (theta,), eps=1e-5, atol=1e-6,
+                                    rtol=1e-5)
+    E = energy(theta, evaluate)
+    (g,) = torch.autograd.grad(2.5 * E, theta)
+    assert np.allclose(g.numpy(), 2.5 * evaluate(THETA1)["grad"], atol=1e-12)
