@@ -10,10 +10,14 @@ measurement interval of the reference cadence.
 Two forms run from identical seeds:
   fp32  W in FP32 with `delay` pending rank-1 terms per flush, FP64 local
         energy, FP64 LU refresh every k accepted moves  -> `value`
-  fp64  W in FP64, immediate rank-1 update (the reference algorithm),
-        FP64 refresh every 216 accepted moves
-`value` is the FP32 form's walker-sweeps/s; the FP64 form, the ratio and
-|dE/site| between the forms are reported beside it.
+  fp64  W in FP64, immediate rank-1 update and the refresh cadence of the
+        reference (src/MonteCarlo.jl:593-604: every min(N_up, N_dn) sweeps
+        that pass the early returns)
+`value` is the FP32 form's walker-sweeps/s.  Beside it: the FP64 form and the
+ratio, the FP32 forms at other refresh intervals, and |dE/site|, the
+difference of the mean local energy computed from the W the FP32 chains carry
+and from a fresh FP64 refresh of the same configurations.  `roofline` covers
+whichever of the sweep and refresh kernels takes longer in the FP32 form.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one
 GPU.
@@ -32,6 +36,13 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+
+# HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) per
+# walker refreshed by k_refresh_blocked, from rocprofv3 --pmc passes over this
+# script's default FP32 run (summaries in profiles/).
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 20_857_570}
+# the same for k_sweep, per walker-sweep (FP32, delay 8, L=12)
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 33_864}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),
@@ -64,20 +75,26 @@ def sweep_bytes(cnt, s, ns, nup, ndn, nw):
 
 
 def refresh_flops(ns, n):
-    """FP64 flops of one spin's refresh in this kernel: elimination of the
-    ns x n matrix [tildeU; U_empty] with pivots from the top n rows, then the
-    unit-lower solve of the ns - n empty rows."""
-    lu = sum((ns - k - 1) + 2 * (ns - k - 1) * (n - k - 1) for k in range(n))
-    solve = (ns - n) * n * (n - 1)
-    return lu + solve
+    """Algorithmic FP64 flops of one spin's refresh W = U tildeU^-1: pivoted
+    LU of the n x n tildeU (2/3 n^3) and the two triangular solves for the ns
+    rows of W (2 ns n^2).  The kernel itself skips the occupied rows of W
+    (they are unit rows), so it executes fewer flops than this count."""
+    return 2.0 * n**3 / 3.0 + 2.0 * ns * n * n
+
+
+def refresh_min_bytes(ns, n, s):
+    """Algorithmic HBM bytes of one spin's refresh: read U (FP64, ns x n) and
+    kappa, write W (s bytes per element)."""
+    return 8 * ns * n + 4 * ns + s * ns * n
 
 
 def run_form(nat, ham, args, precision, delay, k, label):
     ns, n = ham.ns, ham.N_up
     nw = args.walkers
     S = args.sweeps_per_step or n
+    policy = nat.REFRESH_EVERY_K if k > 0 else nat.REFRESH_REFERENCE
     ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, precision=precision,
-                           refresh_policy=nat.REFRESH_EVERY_K, refresh_k=k, delay=delay,
+                           refresh_policy=policy, refresh_k=max(k, 0), delay=delay,
                            seed=args.seed)
     t0 = time.perf_counter()
     redraws = ds.init_random(args.seed)
@@ -105,12 +122,20 @@ def run_form(nat, ham, args, precision, delay, k, label):
     sync()
     wall = time.perf_counter() - t0
     prof = ds.profile()
+    ds.profile(False)
     drift = ds.w_drift() if args.drift else None
+    # Accuracy of the carried W (outside the timed region): local energies
+    # from the W the chains hold now against those from a fresh FP64
+    # refresh of the same configurations.
+    e_carried, _ = ds.measure()
+    ds.refresh()
+    e_fresh, _ = ds.measure()
     ds.close()
     s = 8 if precision == nat.PREC_FP64 else 4
     b_sweep = sweep_bytes(tot, s, ns, n, ham.N_down, nw)
     b_eloc = s * 2 * nap + 8 * ns * nw * args.steps
-    flops_ref = tot["refreshes"] * (refresh_flops(ns, n) + refresh_flops(ns, ham.N_down))
+    flops_walker = refresh_flops(ns, n) + refresh_flops(ns, ham.N_down)
+    flops_ref = tot["refreshes"] * flops_walker
     res = {
         "walker_sweeps_per_s": nw * S * args.steps / wall,
         "ms_per_step": 1e3 * wall / args.steps,
@@ -124,11 +149,21 @@ def run_form(nat, ham, args, precision, delay, k, label):
         "rank1_equiv_gbs": tot["accepted"] * 2 * s * ns * (n + ham.N_down) / (prof["sweep_ms"] * 1e-3) / 1e9,
         "eloc_gbs": b_eloc / (prof["measure_ms"] * 1e-3) / 1e9,
         "refresh_tflops": (flops_ref / (prof["refresh_ms"] * 1e-3) / 1e12) if prof["refresh_ms"] > 0 else None,
+        "refresh_flops_per_walker": flops_walker,
+        "refresh_walkers_per_launch": (tot["refreshes"] / prof["refresh_launches"]) if prof["refresh_launches"] else 0,
+        "refresh_ms_per_launch": (prof["refresh_ms"] / prof["refresh_launches"]) if prof["refresh_launches"] else 0,
+        "sweep_ms_per_launch": prof["sweep_ms"] / max(prof["sweep_launches"], 1),
+        "sweep_bytes_per_launch": b_sweep / max(prof["sweep_launches"], 1),
+        "walker_sweeps_per_launch": nw * S * args.steps / max(prof["sweep_launches"], 1),
         "energy_per_site": float(np.mean(energies) / ns),
         "w_drift": drift,
+        "dE_per_site_carried_vs_refreshed": float(abs(np.mean(e_carried) - np.mean(e_fresh)) / ns),
+        "max_walker_dE_per_site": float(np.abs(e_carried - e_fresh).max() / ns),
     }
     log(f"[{label}] {res['walker_sweeps_per_s']:.3e} walker-sweeps/s, acc {res['accept_rate']:.3f}, "
-        f"sweep {res['sweep_gbs']:.0f} GB/s, kernel ms {res['kernel_ms']}")
+        f"sweep {res['sweep_gbs']:.0f} GB/s, kernel ms {res['kernel_ms']}, refreshes "
+        f"{tot['refreshes']}, dE/site {res['dE_per_site_carried_vs_refreshed']:.2e} "
+        f"(max walker {res['max_walker_dE_per_site']:.2e})")
     return res, np.array(energies)
 
 
@@ -161,28 +196,73 @@ def cpu_baseline(ham, seconds, threads, seed):
     return sum(counts) / wall, sum(counts), wall
 
 
+def copy_calibration(mib):
+    """A plain device copy of `mib` MiB (torch's copy kernel), run a few times
+    so that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be read against a
+    known byte count."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "launches": 3}), flush=True)
+
+
+def roofline_refresh(r, ns, traffic_per_walker):
+    wpl = r["refresh_walkers_per_launch"]
+    if not wpl:
+        return None
+    achieved = r["refresh_flops_per_walker"] * wpl / (r["refresh_ms_per_launch"] * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic_per_walker * wpl if traffic_per_walker else None,
+            "kernel": "k_refresh_blocked (FP64 pivoted LU + MFMA solve, W = U tildeU^-1)",
+            "units_per_launch": wpl, "launch_ms": r["refresh_ms_per_launch"]}
+
+
+def roofline_sweep(r, traffic_per_walker_sweep):
+    achieved = r["sweep_bytes_per_launch"] / (r["sweep_ms_per_launch"] * 1e-3) / 1e9
+    wsl = r["walker_sweeps_per_launch"]
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_per_walker_sweep * wsl if traffic_per_walker_sweep else None,
+            "units_per_launch": wsl,
+            "kernel": "k_sweep (exchange step + delayed Sherman-Morrison updates)",
+            "launch_ms": r["sweep_ms_per_launch"]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--walkers", type=int, default=8192)
     ap.add_argument("--sweeps-per-step", type=int, default=0)
     ap.add_argument("--delay", type=int, default=8)
     ap.add_argument("--k", type=int, default=64, help="FP32 form: accepted moves per refresh")
-    ap.add_argument("--k64", type=int, default=216, help="FP64 form: accepted moves per refresh")
+    ap.add_argument("--k64", type=int, default=0,
+                    help="FP64 form: accepted moves per refresh (0: the reference cadence)")
+    ap.add_argument("--k-extra", default="16,216",
+                    help="further FP32 refresh intervals reported beside the headline")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--drift", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); walkers are "
               "independent chains, so N GPUs would each run this same single-GPU job.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from kagome_vmc import DoubleKagome, Hamiltonian
     from kagome_vmc import _native as nat
@@ -198,6 +278,13 @@ def main():
     r64 = e64 = None
     if not args.skip_fp64:
         r64, e64 = run_form(nat, ham, args, nat.PREC_FP64, 1, args.k64, "fp64")
+    extra = {}
+    for k in [int(x) for x in args.k_extra.split(",") if x.strip()]:
+        rk, _ = run_form(nat, ham, args, nat.PREC_FP32, args.delay, k, f"fp32 k={k}")
+        extra[str(k)] = {key: rk[key] for key in (
+            "walker_sweeps_per_s", "ms_per_step", "kernel_ms", "refresh_tflops",
+            "dE_per_site_carried_vs_refreshed", "max_walker_dE_per_site")}
+        extra[str(k)]["refreshes"] = rk["counters"]["refreshes"]
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -208,6 +295,10 @@ def main():
                          f"L={L}, {threads} chains x {nsw // threads} sweeps in {wall:.1f}s"}
         log(f"[cpu] {rate:.3e} walker-sweeps/s on {threads} threads")
 
+    # the dominant kernel of the headline form carries the roofline
+    rr = roofline_refresh(r32, ham.ns, REFRESH_TRAFFIC_BYTES_PER_WALKER.get(L))
+    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if args.delay == 8 else None)
+    dominant = rr if rr and r32["kernel_ms"]["refresh_ms"] > r32["kernel_ms"]["sweep_ms"] else rs
     value = r32["walker_sweeps_per_s"]
     line = {
         "metric": "walker-sweeps/s at L=12 (FP32-update vs FP64) at |dE/site|<=1e-5; E_loc evals/s",
@@ -221,6 +312,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "dtype_detail": "W and determinant ratios in f32 (delayed rank-1 updates); local energy, "
+                        "refresh LU and solve in f64",
         "data": "synthetic",
         "config": {
             "workload": f"pi-flux Kagome L={L} ({ham.ns} sites, {n}/{n}), {args.walkers} walkers, "
@@ -228,19 +321,18 @@ def main():
                         f"FP64 refresh every {args.k} accepts",
             "L": L, "walkers": args.walkers, "delay": args.delay, "refresh_k": args.k,
         },
-        "roofline": {
-            "bound": "hbm", "achieved": r32["sweep_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": r32["sweep_gbs"] / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_sweep (FP32, delayed rank-1 updates)",
-        },
+        "roofline": dominant,
+        "roofline_other": rs if dominant is rr else rr,
         "cpu_baseline": cpu,
         "fp32": r32,
         "fp64": r64,
+        "fp32_other_k": extra,
     }
+    line["dE_per_site_fp32"] = r32["dE_per_site_carried_vs_refreshed"]
     if r64 is not None:
         line["fp64_walker_sweeps_per_s"] = r64["walker_sweeps_per_s"]
         line["speedup_fp32_vs_fp64"] = value / r64["walker_sweeps_per_s"]
-        line["dE_per_site_fp32_vs_fp64"] = abs(float(np.mean(e32) - np.mean(e64))) / ham.ns
+        line["dE_per_site_fp32_vs_fp64_chains"] = abs(float(np.mean(e32) - np.mean(e64))) / ham.ns
     line["eloc_evals_per_s"] = r32["eloc_evals_per_s_kernel"]
     print(json.dumps(line), flush=True)
 

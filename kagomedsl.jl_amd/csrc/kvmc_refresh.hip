@@ -1,5 +1,5 @@
 // W refresh (reevaluateW!, src/MonteCarlo.jl:55-66) as a blocked FP64
-// factorisation on gfx950: one 512-thread workgroup per (walker, spin).
+// factorisation on gfx950: one 256-thread workgroup per (walker, spin).
 //
 // Work matrix (row-major, zero padded): M = [ tildeU ; U_empty ], ns x N.
 // Physical rows 0..N-1 are U at the occupied sites ordered by label (tildeU,
