@@ -16,7 +16,8 @@ Two forms run from identical seeds:
 `value` is the FP32 form's walker-sweeps/s.  Beside it: the FP64 form and the
 ratio, the FP32 forms at other refresh intervals, and |dE/site|, the
 difference of the mean local energy computed from the W the FP32 chains carry
-and from a fresh FP64 refresh of the same configurations.  `roofline` covers
+and from a fresh FP64 refresh of the same configurations (worst of several
+untimed steps after the timed region).  `roofline` covers
 whichever of the sweep and refresh kernels takes longer in the FP32 form.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one
@@ -124,12 +125,21 @@ def run_form(nat, ham, args, precision, delay, k, label):
     prof = ds.profile()
     ds.profile(False)
     drift = ds.w_drift() if args.drift else None
-    # Accuracy of the carried W (outside the timed region): local energies
-    # from the W the chains hold now against those from a fresh FP64
-    # refresh of the same configurations.
-    e_carried, _ = ds.measure()
-    ds.refresh()
-    e_fresh, _ = ds.measure()
+    # Accuracy of the carried W (outside the timed region): after each of
+    # `probe_steps` further steps, local energies from the W the chains hold
+    # against those of a fresh FP64 refresh of the same configurations (a
+    # second handle); the worst step is reported.
+    d_mean = d_max = 0.0
+    if precision != nat.PREC_FP64 and args.probe_steps > 0:
+        ref = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, precision=nat.PREC_FP64)
+        for _ in range(args.probe_steps):
+            ds.sweep(S)
+            e_carried, _ = ds.measure()
+            ref.set_configs(*ds.get_configs())
+            e_fresh, _ = ref.measure()
+            d_mean = max(d_mean, abs(float(np.mean(e_carried) - np.mean(e_fresh))) / ns)
+            d_max = max(d_max, float(np.abs(e_carried - e_fresh).max()) / ns)
+        ref.close()
     ds.close()
     s = 8 if precision == nat.PREC_FP64 else 4
     b_sweep = sweep_bytes(tot, s, ns, n, ham.N_down, nw)
@@ -157,8 +167,8 @@ def run_form(nat, ham, args, precision, delay, k, label):
         "walker_sweeps_per_launch": nw * S * args.steps / max(prof["sweep_launches"], 1),
         "energy_per_site": float(np.mean(energies) / ns),
         "w_drift": drift,
-        "dE_per_site_carried_vs_refreshed": float(abs(np.mean(e_carried) - np.mean(e_fresh)) / ns),
-        "max_walker_dE_per_site": float(np.abs(e_carried - e_fresh).max() / ns),
+        "dE_per_site_carried_vs_refreshed": d_mean,
+        "max_walker_dE_per_site": d_max,
     }
     log(f"[{label}] {res['walker_sweeps_per_s']:.3e} walker-sweeps/s, acc {res['accept_rate']:.3f}, "
         f"sweep {res['sweep_gbs']:.0f} GB/s, kernel ms {res['kernel_ms']}, refreshes "
@@ -236,22 +246,24 @@ def roofline_sweep(r, traffic_per_walker_sweep):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--walkers", type=int, default=8192)
     ap.add_argument("--sweeps-per-step", type=int, default=0)
     ap.add_argument("--delay", type=int, default=8)
-    ap.add_argument("--k", type=int, default=64, help="FP32 form: accepted moves per refresh")
+    ap.add_argument("--k", type=int, default=128, help="FP32 form: accepted moves per refresh")
     ap.add_argument("--k64", type=int, default=0,
                     help="FP64 form: accepted moves per refresh (0: the reference cadence)")
-    ap.add_argument("--k-extra", default="16,216",
+    ap.add_argument("--k-extra", default="16,64,216",
                     help="further FP32 refresh intervals reported beside the headline")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--drift", action="store_true")
+    ap.add_argument("--probe-steps", type=int, default=4,
+                    help="untimed steps after the timed region, each checking the carried W")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()

@@ -110,7 +110,10 @@ int kvmc_get_configs(kvmc_handle* h, int32_t* kappa_up, int32_t* kappa_down);
 /* Seeded uniform Sz=0 start for every walker (replaces the QR start of
  * Carlo.init!, src/MonteCarlo.jl:435-441): Philox Fisher-Yates permutation
  * keyed (seed, walker, attempt); walkers whose tildeU is singular are
- * redrawn up to max_attempts times.  *redraws receives the number of redraws. */
+ * redrawn up to max_attempts times.  Singular here means pivots of the FP64
+ * LU spanning more than 1e10 (FP64 handles) or 1e4 (FP32 handles, whose
+ * float W could not resolve the ratios).  *redraws receives the number of
+ * redraws. */
 int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_t* redraws);
 
 /* reevaluateW! for every walker (src/MonteCarlo.jl:55-66): FP64 pivoted LU

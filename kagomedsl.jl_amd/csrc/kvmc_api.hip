@@ -22,8 +22,13 @@ namespace {
 thread_local std::string g_err;
 
 // Random starts whose pivots span more than this ratio are redrawn: such a
-// tildeU is numerically singular even when no pivot is exactly zero.
+// tildeU is numerically singular even when no pivot is exactly zero.  With W
+// stored in float the bar is higher: a start whose pivots span more than 1e4
+// would carry a float W with fewer than ~10 significant bits in its ratios.
+// (At L = 12 the pivot ratios of uniform starts are bimodal, 1e-15 or
+// above 1e-3, so the stricter bar costs about 2 % more redraws.)
 constexpr double kStartPivotTol = 1e-10;
+constexpr double kStartPivotTolF32 = 1e-4;
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -436,7 +441,8 @@ int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_
     int rc = upload_configs(h, ku.data(), kd.data());
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(h->status, 0, sizeof(int) * h->nw, h->stream));
-    rc = refresh_list(h, h->all_walkers, h->nw, 1, kStartPivotTol);
+    const double start_tol = h->prec == KVMC_PREC_FP32 ? kStartPivotTolF32 : kStartPivotTol;
+    rc = refresh_list(h, h->all_walkers, h->nw, 1, start_tol);
     if (rc != KVMC_OK && rc != KVMC_ERR_SINGULAR) return rc;
     int64_t nredraw = 0;
     std::vector<int> status(h->nw);
@@ -461,7 +467,7 @@ int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_
         HIP_TRY(hipMemcpyAsync(dbad, bad.data(), bad.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
         for (int w : bad) status[w] = 0;
         HIP_TRY(hipMemcpyAsync(h->status, status.data(), sizeof(int) * h->nw, hipMemcpyHostToDevice, h->stream));
-        rc = refresh_list(h, dbad, (int)bad.size(), 1, kStartPivotTol);
+        rc = refresh_list(h, dbad, (int)bad.size(), 1, start_tol);
         HIP_TRY(hipStreamSynchronize(h->stream));
         (void)hipFree(dbad);
         if (rc != KVMC_OK && rc != KVMC_ERR_SINGULAR) return rc;

@@ -232,20 +232,12 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
         if (t == 0) { a.status[w] = 2; atomicMin(a.failed, w); }
         return;
     }
-    for (int r0 = 4 * wave; r0 < nrows; r0 += 4 * NWAVE) {
-        for (int c = lane; c < ldm; c += 64) {
-            double v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int r = r0 + u;
-                v[u] = (r < ns && c < N) ? UT[(size_t)rowsite[r] * N + c] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (r0 + u < nrows) M[(size_t)(r0 + u) * ldm + c] = v[u];
-        }
-    }
-    __syncthreads();
+    // M is never filled from U up front: the first panel (j0 = 0) reads its
+    // operands straight from U (L2-resident, shared by every job) and every
+    // later read of M finds values written by an earlier step.
+    auto u_at = [&](int prow, int col) -> double {
+        return (prow < ns && col < N) ? UT[(size_t)rowsite[prow] * N + col] : 0.0;
+    };
 
     // ---- 1. blocked LU ---------------------------------------------------
     bool singular = false;
@@ -262,7 +254,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
             myp[q] = lr < N ? phys[lr] : -1;
 #pragma unroll
             for (int c = 0; c < RB; ++c)
-                x[q][c] = (myp[q] >= 0) ? M[(size_t)myp[q] * ldm + j0 + c] : 0.0;
+                x[q][c] = (myp[q] < 0) ? 0.0 : (j0 == 0 ? u_at(myp[q], c) : M[(size_t)myp[q] * ldm + j0 + c]);
         }
 #pragma clang loop unroll(full)
         for (int jj = 0; jj < RB; ++jj) {
@@ -352,7 +344,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
                 for (int s2 = 0; s2 < RB / 4; ++s2) {
                     const int k = 4 * s2 + (lane >> 4);
-                    const double av = M[(size_t)ra * ldm + j0 + k];
+                    const double av = j0 == 0 ? u_at(ra, k) : M[(size_t)ra * ldm + j0 + k];
                     y0 = mfma16(av, Linv[k * LB + (lane & 15)], y0);
                     y1 = mfma16(av, Linv[k * LB + 16 + (lane & 15)], y1);
                 }
@@ -384,7 +376,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
             for (int s = 0; s < RB / 4; ++s) {
                 const int k = 4 * s + (lane >> 4);
-                const double b = M[(size_t)phys[j0 + k] * ldm + c0 + cc];
+                const double b = j0 == 0 ? u_at(phys[k], c0 + cc) : M[(size_t)phys[j0 + k] * ldm + c0 + cc];
                 u0 = mfma16(Linv[(lane & 15) * LB + k], b, u0);
                 u1 = mfma16(Linv[(16 + (lane & 15)) * LB + k], b, u1);
             }
@@ -407,17 +399,24 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
             for (int s = 0; s < RB / 4; ++s) af[s] = -M[(size_t)pa * ldm + j0 + 4 * s + (lane >> 4)];
             double* crow[4];
+            int prow[4];
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
-                crow[g] = M + (size_t)phys[r0 + 16 * ti + (lane >> 4) + 4 * g] * ldm + c0 + (lane & 15);
+            for (int g = 0; g < 4; ++g) {
+                prow[g] = phys[r0 + 16 * ti + (lane >> 4) + 4 * g];
+                crow[g] = M + (size_t)prow[g] * ldm + c0 + (lane & 15);
+            }
+            // C operands: from U in the first panel, from M afterwards
+            auto cval = [&](int g, int tj) -> double {
+                return j0 == 0 ? u_at(prow[g], c0 + 16 * tj + (lane & 15)) : crow[g][16 * tj];
+            };
             f64x4 acc, nxt;
 #pragma unroll
-            for (int g = 0; g < 4; ++g) acc[g] = crow[g][0];
+            for (int g = 0; g < 4; ++g) acc[g] = cval(g, 0);
             for (int tj = 0; tj < ntc; ++tj) {
                 const int cc = 16 * tj + (lane & 15);
                 if (tj + 1 < ntc) {
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) nxt[g] = crow[g][16 * (tj + 1)];
+                    for (int g = 0; g < 4; ++g) nxt[g] = cval(g, tj + 1);
                 }
 #pragma unroll
                 for (int s = 0; s < RB / 4; ++s)
