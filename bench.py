@@ -96,7 +96,7 @@ def run_form(nat, ham, args, precision, delay, k, label):
     policy = nat.REFRESH_EVERY_K if k > 0 else nat.REFRESH_REFERENCE
     ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, precision=precision,
                            refresh_policy=policy, refresh_k=max(k, 0), delay=delay,
-                           seed=args.seed)
+                           threads=args.threads, seed=args.seed)
     t0 = time.perf_counter()
     redraws = ds.init_random(args.seed)
     log(f"[{label}] init {nw} walkers L={args.L}: {time.perf_counter() - t0:.2f}s "
@@ -252,6 +252,7 @@ def main():
     ap.add_argument("--walkers", type=int, default=8192)
     ap.add_argument("--sweeps-per-step", type=int, default=0)
     ap.add_argument("--delay", type=int, default=8)
+    ap.add_argument("--threads", type=int, default=0, help="sweep-kernel workgroup size (0: automatic)")
     ap.add_argument("--k", type=int, default=128, help="FP32 form: accepted moves per refresh")
     ap.add_argument("--k64", type=int, default=0,
                     help="FP64 form: accepted moves per refresh (0: the reference cadence)")

@@ -265,7 +265,9 @@ int upload_configs(kvmc_handle* h, const int32_t* ku, const int32_t* kd) {
 int sweep_threads(const kvmc_handle* h) {
     const int V = h->prec == 0 ? 2 : 4;
     const int nvec = (h->ns % V == 0) ? h->ns / V : h->ns;
-    int t = h->threads > 0 ? h->threads : 256;
+    // default: one wavefront when a column fits in it (more walkers per CU
+    // hide the proposal latency), else 256 threads
+    int t = h->threads > 0 ? h->threads : (nvec <= 64 ? 64 : 256);
     t = std::max(t, (nvec + 63) / 64 * 64);
     return std::min(t, 512);
 }

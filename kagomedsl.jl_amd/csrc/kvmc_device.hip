@@ -14,7 +14,9 @@
 //
 // Layout in HBM (per walker, walker-major): kappa_up[ns], kappa_dn[ns]
 // (int32, 1-based labels, 0 = empty); W_up (ns x nup) then W_dn (ns x ndn),
-// column-major, so column l (a label) is contiguous.
+// column-major, so column l (a label) is contiguous.  For small clusters
+// (both W of a walker within kResidentMaxBytes) the sweep kernel copies W
+// into LDS at launch and back at exit, so sweeps and flushes never touch HBM.
 #include "kvmc_internal.hpp"
 
 #include <float.h>
@@ -152,7 +154,13 @@ __device__ inline double block_sum_double(double v, double* red) {
 // same (uniform) control flow; proposals are evaluated redundantly so no
 // barrier is needed on rejected sweeps.
 // ---------------------------------------------------------------------------
-template <typename T, int V, int DMAX>
+__host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay) {
+    const size_t nsp = (size_t)(ns + 3) / 4 * 4;
+    const size_t b = 2 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double);
+    return (b + 15) / 16 * 16;
+}
+
+template <typename T, int V, int DMAX, bool RES>
 __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int w = a.walker0 + (int)blockIdx.x;
@@ -166,7 +174,9 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
 
     const size_t wstride = (size_t)ns * (nup + ndn);
-    T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
+    T* const Wg = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
+    // W of this walker: in HBM, or (RES) an LDS copy for the whole launch
+    T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nmax, delay)) : Wg;
     T* Wdn = Wup + (size_t)ns * nup;
     T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsp;
     T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
@@ -188,6 +198,9 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     if (c > 0) {
         for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
         for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
+    }
+    if (RES) {
+        for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wup[t] = Wg[t];
     }
     __syncthreads();
     // Z_mu: full recount once per launch, then maintained incrementally.
@@ -292,6 +305,9 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
         a.kup[(size_t)w * ns + t] = s_kup[t];
         a.kdn[(size_t)w * ns + t] = s_kdn[t];
     }
+    if (RES) {
+        for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wg[t] = Wup[t];
+    }
     if (c > 0) {
         for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) gA[t] = s_A[t];
         for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) gB[t] = s_B[t];
@@ -312,23 +328,30 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     }
 }
 
+bool sweep_resident(int prec, int ns, int nup, int ndn) {
+    const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+    return (size_t)ns * (nup + ndn) * es <= kResidentMaxBytes;
+}
+
 size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay) {
-    const int nsp = (ns + 3) / 4 * 4;
     const int nmax = nup > ndn ? nup : ndn;
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
-    return (size_t)2 * nsp * sizeof(int) + (size_t)2 * delay * (nsp + nmax) * es +
-           16 * sizeof(double);
+    return sweep_lds_base(es, ns, nmax, delay) +
+           (sweep_resident(prec, ns, nup, ndn) ? (size_t)ns * (nup + ndn) * es : 0);
 }
 
 template <typename T, int V, int D>
 static hipError_t launch_sweep_d(const SweepArgs& a, int nw, int threads, size_t lds,
                                  hipStream_t s) {
+    const bool res = sweep_resident(sizeof(T) == 8 ? 0 : 1, a.ns, a.nup, a.ndn);
+    const void* fn = res ? reinterpret_cast<const void*>(&k_sweep<T, V, D, true>)
+                         : reinterpret_cast<const void*>(&k_sweep<T, V, D, false>);
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep<T, V, D>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sweep<T, V, D>), dim3(nw), dim3(threads), lds, s, a);
+    if (res) hipLaunchKernelGGL((k_sweep<T, V, D, true>), dim3(nw), dim3(threads), lds, s, a);
+    else hipLaunchKernelGGL((k_sweep<T, V, D, false>), dim3(nw), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 

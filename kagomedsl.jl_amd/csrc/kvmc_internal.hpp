@@ -127,6 +127,9 @@ struct ElocArgs {
 hipError_t launch_sweep(int prec, const SweepArgs& a, int nwalkers, int threads, size_t lds,
                         hipStream_t s);
 size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay);
+// Both W of a walker at most this large: the sweep kernel keeps them in LDS.
+constexpr size_t kResidentMaxBytes = 64 * 1024;
+bool sweep_resident(int prec, int ns, int nup, int ndn);
 hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s);
 size_t refresh_ws_doubles_per_job(int ns, int nmax);
 bool refresh_supported(int ns, int nup, int ndn);
