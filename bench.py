@@ -243,6 +243,54 @@ def roofline_sweep(r, traffic_per_walker_sweep):
             "launch_ms": r["sweep_ms_per_launch"]}
 
 
+MIXED_GROUPS = ((4, 4096), (6, 2048), (8, 2048), (10, 1024), (12, 1024))
+
+
+def run_mixed(args):
+    """All groups of MIXED_GROUPS (L, walkers) at once, one handle and host
+    thread per group, FP32 W with delay and FP64 refresh every `k` accepts.
+    One step = n_occupied sweeps + one local-energy pass in every group.
+    Also times each group alone, for the overlap gain."""
+    from kagome_vmc import _native as nat
+    from kagome_vmc.batch import MixedBatch, pi_flux_cluster
+    nat.load()
+    groups = [(pi_flux_cluster(L), nw) for L, nw in MIXED_GROUPS]
+    kw = dict(precision=nat.PREC_FP32, refresh_policy=nat.REFRESH_EVERY_K, refresh_k=args.k,
+              delay=args.delay, seed=args.seed)
+
+    def timed(batch):
+        batch.init_random(args.seed)
+        batch.step(args.warmup)
+        sync()
+        t0 = time.perf_counter()
+        res = batch.step(args.steps)
+        sync()
+        return time.perf_counter() - t0, res
+
+    with MixedBatch(groups, **kw) as mb:
+        wall, res = timed(mb)
+    per = []
+    alone_total = 0.0
+    for (ham, nw), (cnt, _) in zip(groups, res):
+        with MixedBatch([(ham, nw)], **kw) as one:
+            t1, _ = timed(one)
+        alone_total += t1
+        per.append({"L": int(round((ham.ns / 3) ** 0.5)), "walkers": nw, "walker_sweeps": cnt["sweeps"],
+                    "alone_walker_sweeps_per_s": cnt["sweeps"] / t1, "refreshes": cnt["refreshes"]})
+    total = sum(c["sweeps"] for c, _ in res)
+    line = {
+        "metric": "walker-sweeps/s, mixed cluster sizes L in {4,6,8,10,12} on one GPU",
+        "value": total / wall, "unit": "walker-sweeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "mixed: " + ", ".join(f"L={L} x {nw}" for L, nw in MIXED_GROUPS)
+                               + f"; FP32 W delay {args.delay}, FP64 refresh every {args.k} accepts",
+                   "groups": per},
+        "sequential_walker_sweeps_per_s": total / alone_total,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +313,8 @@ def main():
     ap.add_argument("--drift", action="store_true")
     ap.add_argument("--probe-steps", type=int, default=4,
                     help="untimed steps after the timed region, each checking the carried W")
+    ap.add_argument("--mixed", action="store_true",
+                    help="only run the mixed-cluster-size batch (L = 4..12 groups at once)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -275,6 +325,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.mixed:
+        run_mixed(args)
         return
 
     from kagome_vmc import DoubleKagome, Hamiltonian

@@ -1,0 +1,104 @@
+"""Walker groups of different cluster sizes on one GPU.
+
+Each group is one device handle (its own HIP stream and buffers); a host
+thread per group drives its sweeps, so the groups' kernels overlap on the
+GPU.  Groups are independent Markov chains, so a group's results do not
+depend on which other groups run beside it.
+"""
+from __future__ import annotations
+
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as nat
+from .hamiltonian import Hamiltonian, Hmat, shell_gap
+from .lattice import DoubleKagome
+
+# antiPBC choices tried in order for a closed shell at half filling
+_BC_CANDIDATES = ((False, True), (False, False), (True, False), (True, True))
+
+
+def pi_flux_cluster(L: int, n_up: int | None = None) -> Hamiltonian:
+    """The L x L pi-flux cluster (3 L^2 sites, periodic in both directions)
+    at half filling, with the first antiperiodic choice that closes the
+    shell (the Slater determinant is then unique)."""
+    n = 3 * L * L // 2 if n_up is None else n_up
+    for apbc in _BC_CANDIDATES:
+        lat = DoubleKagome.build(1.0, L, L, (True, True), apbc)
+        w = np.linalg.eigvalsh(Hmat(lat).real)
+        if shell_gap(w, n) > 1e-8 and shell_gap(w, 3 * L * L - n) > 1e-8:
+            return Hamiltonian.build(n, 3 * L * L - n, lat)
+    raise ValueError(f"no boundary condition closes the shell at L={L}, N_up={n}")
+
+
+@dataclass
+class Group:
+    ham: Hamiltonian
+    sampler: nat.DeviceSampler
+
+    @property
+    def n_occupied(self) -> int:
+        return min(self.ham.N_up, self.ham.N_down)
+
+
+class MixedBatch:
+    """groups: iterable of (Hamiltonian, nwalkers).  Sampler keywords
+    (precision, refresh_policy, refresh_k, delay, kernel, seed) apply to all
+    groups; group g draws with seed + g."""
+
+    def __init__(self, groups, **sampler_kw):
+        seed = int(sampler_kw.pop("seed", 0))
+        self.groups = []
+        for g, (ham, nw) in enumerate(groups):
+            ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, int(nw),
+                                   seed=seed + g, **sampler_kw)
+            self.groups.append(Group(ham, ds))
+
+    def _parallel(self, fn):
+        out = [None] * len(self.groups)
+        err = []
+
+        def run(i):
+            try:
+                out[i] = fn(self.groups[i])
+            except Exception as e:  # re-raised in the caller's thread
+                err.append(e)
+
+        ths = [threading.Thread(target=run, args=(i,)) for i in range(len(self.groups))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        if err:
+            raise err[0]
+        return out
+
+    def init_random(self, seed: int):
+        return self._parallel(lambda g: g.sampler.init_random(seed))
+
+    def step(self, nsteps: int = 1):
+        """nsteps measurement intervals on every group: n_occupied sweeps of
+        each walker, then its local energies.  Returns per group the summed
+        counters and the list of per-walker energy arrays."""
+        def run(g):
+            tot = dict(sweeps=0, moves=0, accepted=0, refreshes=0, flushes=0, launches=0)
+            energies = []
+            for _ in range(nsteps):
+                cnt, _ = g.sampler.sweep(g.n_occupied)
+                for k in tot:
+                    tot[k] += cnt[k]
+                energies.append(g.sampler.measure()[0])
+            return tot, energies
+        return self._parallel(run)
+
+    def close(self):
+        for g in self.groups:
+            g.sampler.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
