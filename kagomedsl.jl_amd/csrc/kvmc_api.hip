@@ -524,7 +524,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
     a.tape_u = d_tu; a.tape_bond = d_tb; a.tape_len = nsweeps; a.outcomes = d_out;
     a.walker0 = 0;
     const int threads = sweep_threads(h);
-    const size_t lds = sweep_lds_bytes(h->prec, h->ns, h->nup, h->ndn, h->delay);
+    const size_t lds = sweep_lds_bytes(h->prec, h->ns, h->nup, h->ndn, h->delay, h->nb);
     if (lds > 160 * 1024) { cleanup(); return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay"); }
     int64_t launches = 0, refreshes = 0;
     hipStream_t s = h->stream;

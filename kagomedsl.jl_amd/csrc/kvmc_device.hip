@@ -43,12 +43,11 @@ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // lanes); its A_q slices stay in registers for the whole pass, B_q[j] is an
 // LDS broadcast.  Requires blockDim.x >= ns / V (checked on the host).
 // ---------------------------------------------------------------------------
-template <typename T, int V, int DMAX>
+template <typename T, int V, int DMAX, int UNR = 4>
 __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __restrict__ sA,
                                               const T* __restrict__ sB, int ns, int nsp,
                                               int nmax, int ncols, int c) {
-    using VT = typename VecT<T, V>::type;
-    constexpr int UNR = 4;   // columns in flight per thread
+    using VT = typename VecT<T, V>::type;   // UNR: columns in flight per thread
     const int nvec = ns / V;
     const int lanes = (int)blockDim.x / nvec;
     const int t = (int)threadIdx.x;
@@ -93,47 +92,79 @@ __device__ __forceinline__ T w_cur(const T* __restrict__ W0, const T* sA, const 
     return v;
 }
 
-// Appends the rank-1 term of an accepted move on one spin (row K, label l,
-// both 0-based): A_c = W[:, l], B_c = alpha * (W[K, :] - e_l), alpha = -1/den
-// with den = W[K, l], exactly the reference's update_W! factors.
+// Appends both spins' rank-1 terms of an accepted move (rows K, labels l,
+// 0-based): A_c = W[:, l], B_c = alpha (W[K, :] - e_l), alpha = -1/W[K, l],
+// exactly the reference's update_W! factors (src/MonteCarlo.jl:279-292).
+// The column and row elements of both W are loaded before any correction or
+// LDS store, so the accept costs one memory round trip.
 template <typename T>
-__device__ __forceinline__ void append_term(const T* __restrict__ W0, T* sA, T* sB, int ns,
-                                            int nsp, int nmax, int ncols, int K, int l, int c,
-                                            T den) {
-    const T alpha = T(-1) / den;
-    T* An = sA + c * nsp;
-    T* Bn = sB + c * nmax;
-    for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
-        T v = W0[(size_t)l * ns + t];
-        for (int q = 0; q < c; ++q) v += sA[q * nsp + t] * sB[q * nmax + l];
-        An[t] = v;
-    }
-    for (int t = (int)threadIdx.x; t < ncols; t += (int)blockDim.x) {
-        T v = W0[(size_t)t * ns + K];
-        for (int q = 0; q < c; ++q) v += sA[q * nsp + K] * sB[q * nmax + t];
-        if (t == l) v -= T(1);
-        Bn[t] = alpha * v;
+__device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T* __restrict__ Wdn,
+                                             T* sAu, T* sBu, T* sAd, T* sBd, int ns, int nsp,
+                                             int nmax, int nup, int ndn, int Kup, int lup, int Kdn,
+                                             int ldn, int c, T wu, T wd) {
+    constexpr int CH = 4;
+    const int bd = (int)blockDim.x, t0 = (int)threadIdx.x;
+    const int nmx = ns > nmax ? ns : nmax;
+    for (int base = 0; base < nmx; base += CH * bd) {
+        T cu[CH], cd[CH], ru[CH], rd[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int t = base + t0 + u * bd;
+            cu[u] = t < ns ? Wup[(size_t)lup * ns + t] : T(0);
+            cd[u] = t < ns ? Wdn[(size_t)ldn * ns + t] : T(0);
+            ru[u] = t < nup ? Wup[(size_t)t * ns + Kup] : T(0);
+            rd[u] = t < ndn ? Wdn[(size_t)t * ns + Kdn] : T(0);
+        }
+        const T au = T(-1) / wu, ad = T(-1) / wd;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int t = base + t0 + u * bd;
+            if (t < ns) {
+                T vu = cu[u], vd = cd[u];
+                for (int q = 0; q < c; ++q) {
+                    vu += sAu[q * nsp + t] * sBu[q * nmax + lup];
+                    vd += sAd[q * nsp + t] * sBd[q * nmax + ldn];
+                }
+                sAu[c * nsp + t] = vu;
+                sAd[c * nsp + t] = vd;
+            }
+            if (t < nup) {
+                T v = ru[u];
+                for (int q = 0; q < c; ++q) v += sAu[q * nsp + Kup] * sBu[q * nmax + t];
+                if (t == lup) v -= T(1);
+                sBu[c * nmax + t] = au * v;
+            }
+            if (t < ndn) {
+                T v = rd[u];
+                for (int q = 0; q < c; ++q) v += sAd[q * nsp + Kdn] * sBd[q * nmax + t];
+                if (t == ldn) v -= T(1);
+                sBd[c * nmax + t] = ad * v;
+            }
+        }
     }
 }
 
-// Antiparallel bonds among those touching site i or site j (the only bonds
-// whose status an exchange of i and j can change).  Matches Z's test
-// (src/MonteCarlo.jl:460-474).
-__device__ __forceinline__ int count_touching(const int2* __restrict__ bonds,
-                                              const int* __restrict__ nbr_off,
-                                              const int* __restrict__ nbr, const int* s_kup,
-                                              const int* s_kdn, int i, int j) {
-    int cnt = 0;
-    for (int p = nbr_off[i]; p < nbr_off[i + 1]; ++p) {
-        int2 b = bonds[nbr[p]];
-        cnt += (s_kup[b.x] != 0 && s_kdn[b.y] != 0) || (s_kup[b.y] != 0 && s_kdn[b.x] != 0);
+// Change of Z (src/MonteCarlo.jl:460-474) when the electrons on the
+// antiparallel pair (i, j) are exchanged: at half filling every site holds
+// one electron, so each bond touching exactly one of i and j flips its
+// antiparallel status, while bonds between i and j and on-site bonds keep
+// theirs.  dZ = (flipping bonds) - 2 (those antiparallel before).
+__device__ __forceinline__ int z_delta(const int2* bonds, const int* __restrict__ nbr_off,
+                                       const int* __restrict__ nbr, const int* s_kup,
+                                       const int* s_kdn, int i, int j) {
+    int flip = 0, ap = 0;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int x = side ? j : i, other = side ? i : j;
+        for (int p = nbr_off[x]; p < nbr_off[x + 1]; ++p) {
+            const int2 b = bonds[nbr[p]];
+            const int y = b.x == x ? b.y : b.x;
+            if (b.x == b.y || y == other) continue;
+            ++flip;
+            ap += (s_kup[b.x] != 0 && s_kdn[b.y] != 0) || (s_kup[b.y] != 0 && s_kdn[b.x] != 0);
+        }
     }
-    for (int p = nbr_off[j]; p < nbr_off[j + 1]; ++p) {
-        int2 b = bonds[nbr[p]];
-        if (b.x == i || b.y == i) continue;
-        cnt += (s_kup[b.x] != 0 && s_kdn[b.y] != 0) || (s_kup[b.y] != 0 && s_kdn[b.x] != 0);
-    }
-    return cnt;
+    return flip - 2 * ap;
 }
 
 __device__ inline double block_sum_double(double v, double* red) {
@@ -154,14 +185,15 @@ __device__ inline double block_sum_double(double v, double* red) {
 // same (uniform) control flow; proposals are evaluated redundantly so no
 // barrier is needed on rejected sweeps.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay) {
+__host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay, int nb) {
     const size_t nsp = (size_t)(ns + 3) / 4 * 4;
-    const size_t b = 2 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double);
+    const size_t b = 2 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
+                     (size_t)nb * sizeof(int2);
     return (b + 15) / 16 * 16;
 }
 
 template <typename T, int V, int DMAX, bool RES>
-__global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k_sweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int w = a.walker0 + (int)blockIdx.x;
     const int ns = a.ns, nup = a.nup, ndn = a.ndn, nb = a.nb, nmax = a.nmax;
@@ -172,11 +204,12 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     T* s_A = reinterpret_cast<T*>(s_kdn + nsp);           // [2][delay][nsp]
     T* s_B = s_A + 2 * delay * nsp;                       // [2][delay][nmax]
     double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
+    int2* s_bond = reinterpret_cast<int2*>(s_red + 16);   // [nb] bond table
 
     const size_t wstride = (size_t)ns * (nup + ndn);
     T* const Wg = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
     // W of this walker: in HBM, or (RES) an LDS copy for the whole launch
-    T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nmax, delay)) : Wg;
+    T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nmax, delay, nb)) : Wg;
     T* Wdn = Wup + (size_t)ns * nup;
     T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsp;
     T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
@@ -202,6 +235,7 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     if (RES) {
         for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wup[t] = Wg[t];
     }
+    for (int b = (int)threadIdx.x; b < nb; b += (int)blockDim.x) s_bond[b] = a.bonds[b];
     __syncthreads();
     // Z_mu: full recount once per launch, then maintained incrementally.
     int zmu;
@@ -218,87 +252,115 @@ __global__ void __launch_bounds__(512) k_sweep(SweepArgs a) {
     const long long tbase = a.tape_base[w];
     unsigned long long n_moves = 0, n_acc = 0, n_flush = 0, n_sweeps = 0;
     bool pause = false;
+    const int lane = (int)threadIdx.x & 63;
 
-    while (sweeps < stop) {
-        double r;
-        int bidx;
-        const long long tidx = sweeps - tbase;
-        if (a.tape_u) {
-            r = a.tape_u[(size_t)w * a.tape_len + tidx];
-            bidx = a.tape_bond[(size_t)w * a.tape_len + tidx];
-        } else {
-            sweep_draws(a.seed, (uint32_t)w, sweeps, nb, &r, &bidx);
+    // Between two accepted moves the walker's state (kappa, Z_mu, W and the
+    // pending terms) does not change, so every proposal up to and including
+    // the next accept is a function of that state and its own draws.  Each
+    // wavefront therefore evaluates a window of 64 consecutive sweeps, one
+    // per lane (all W gathers of the window in flight together), and the
+    // walker advances to the first sweep that accepts or triggers a refresh.
+    // Every wavefront evaluates the same window, so the result is uniform
+    // without a barrier.  Decisions equal those of one-at-a-time evaluation.
+    while (sweeps < stop && !pause) {
+        const long long sw = sweeps + lane;
+        const bool valid = sw < stop;
+        double r = 1.0;
+        int bidx = 0;
+        if (valid) {
+            if (a.tape_u) {
+                const long long tidx = sw - tbase;
+                r = a.tape_u[(size_t)w * a.tape_len + tidx];
+                bidx = a.tape_bond[(size_t)w * a.tape_len + tidx];
+            } else {
+                sweep_draws(a.seed, (uint32_t)w, sw, nb, &r, &bidx);
+            }
         }
         const double zr = (double)zmu / zmax;
-        int oc;
-        bool refresh_due = false;
-        if (a.kernel_mode == KERNEL_REFERENCE && r > zr) {
-            oc = OUT_PRE_REJECT;
-        } else {
-            const int2 bd = a.bonds[bidx];
-            const int i = bd.x, site = bd.y;
+        int oc = OUT_PRE_REJECT;
+        bool mv = false, acc = false;
+        int i = 0, site = 0, lup = 0, ldn = 0, Kup = 0, Kdn = 0, dz = 0;
+        bool can1 = false;
+        T wu = T(0), wd = T(0);
+        if (valid && !(a.kernel_mode == KERNEL_REFERENCE && r > zr)) {
+            const int2 bd = s_bond[bidx];
+            i = bd.x;
+            site = bd.y;
             const int iu = s_kup[i], su = s_kup[site], id = s_kdn[i], sd = s_kdn[site];
-            const bool can1 = iu != 0 && sd != 0;
+            can1 = iu != 0 && sd != 0;
             const bool can2 = su != 0 && id != 0;
             if (!can1 && !can2) {
                 oc = OUT_NO_MOVE;
             } else {
-                ++n_moves;
+                mv = true;
                 // flag = 1: up moves i -> site, down moves site -> i
-                const int lup = (can1 ? iu : su) - 1;
-                const int ldn = (can1 ? sd : id) - 1;
-                const int Kup = can1 ? site : i;
-                const int Kdn = can1 ? i : site;
-                const T wu = w_cur(Wup, sAu, sBu, ns, nsp, nmax, Kup, lup, c);
-                const T wd = w_cur(Wdn, sAd, sBd, ns, nsp, nmax, Kdn, ldn, c);
+                lup = (can1 ? iu : su) - 1;
+                ldn = (can1 ? sd : id) - 1;
+                Kup = can1 ? site : i;
+                Kdn = can1 ? i : site;
+                wu = w_cur(Wup, sAu, sBu, ns, nsp, nmax, Kup, lup, c);
+                wd = w_cur(Wdn, sAd, sBd, ns, nsp, nmax, Kdn, ldn, c);
+                dz = z_delta(s_bond, a.nbr_off, a.nbr, s_kup, s_kdn, i, site);
                 const T ratio = wu * wd;
                 const double acc_prob = (double)ratio * (double)ratio;
-                bool accept;
                 if (a.kernel_mode == KERNEL_REFERENCE)
-                    accept = (acc_prob >= 1.0 && r < zr) || (acc_prob < 1.0 && r < zr * acc_prob);
+                    acc = (acc_prob >= 1.0 && r < zr) || (acc_prob < 1.0 && r < zr * acc_prob);
                 else
-                    accept = r < acc_prob;
-                oc = accept ? OUT_ACCEPT : OUT_REJECT;
-                if (accept) {
-                    ++n_acc;
-                    ++acc_since;
-                    const int zb = count_touching(a.bonds, a.nbr_off, a.nbr, s_kup, s_kdn, i, site);
-                    append_term(Wup, sAu, sBu, ns, nsp, nmax, nup, Kup, lup, c, wu);
-                    append_term(Wdn, sAd, sBd, ns, nsp, nmax, ndn, Kdn, ldn, c, wd);
-                    __syncthreads();
-                    if (threadIdx.x == 0) {
-                        if (can1) {
-                            s_kup[i] = 0; s_kup[site] = lup + 1;
-                            s_kdn[i] = ldn + 1; s_kdn[site] = 0;
-                        } else {
-                            s_kup[i] = lup + 1; s_kup[site] = 0;
-                            s_kdn[i] = 0; s_kdn[site] = ldn + 1;
-                        }
-                    }
-                    ++c;
-                    if (c == delay) {
-                        stream_update<T, V, DMAX>(Wup, sAu, sBu, ns, nsp, nmax, nup, c);
-                        stream_update<T, V, DMAX>(Wdn, sAd, sBd, ns, nsp, nmax, ndn, c);
-                        c = 0;
-                        ++n_flush;
-                    }
-                    __syncthreads();
-                    const int za = count_touching(a.bonds, a.nbr_off, a.nbr, s_kup, s_kdn, i, site);
-                    zmu += za - zb;
-                }
-                if (a.policy == POLICY_REFERENCE)
-                    refresh_due = (sweeps % a.nocc) == 0;
-                else
-                    refresh_due = accept && acc_since >= a.refresh_k;
+                    acc = r < acc_prob;
+                oc = acc ? OUT_ACCEPT : OUT_REJECT;
             }
         }
-        if (a.outcomes && threadIdx.x == 0) a.outcomes[(size_t)w * a.tape_len + tidx] = (signed char)oc;
-        ++sweeps;
-        ++n_sweeps;
-        if (refresh_due) {
-            pause = true;
-            break;
+        // terminal sweep: an accept, or a move on which the reference cadence refreshes
+        const bool term = acc || (mv && a.policy == POLICY_REFERENCE && (sw % a.nocc) == 0);
+        const unsigned long long tmask = __ballot(valid && term);
+        const int nvalid = __popcll(__ballot(valid));
+        const int ncommit = tmask ? __ffsll((long long)tmask) : nvalid;   // sweeps consumed
+        if (a.outcomes && threadIdx.x < 64 && lane < ncommit)
+            a.outcomes[(size_t)w * a.tape_len + (sw - tbase)] = (signed char)oc;
+        n_moves += __popcll(__ballot(mv && lane < ncommit));
+        sweeps += ncommit;
+        n_sweeps += ncommit;
+        if (!tmask) continue;
+        const int tl = ncommit - 1;   // lane of the terminal sweep
+        const bool t_acc = __shfl((int)acc, tl, 64) != 0;
+        const long long t_sw = sweeps - 1;
+        bool refresh_due;
+        if (t_acc) {
+            const int ti = __shfl(i, tl, 64), tsite = __shfl(site, tl, 64);
+            const bool tcan1 = __shfl((int)can1, tl, 64) != 0;
+            const int tlup = __shfl(lup, tl, 64), tldn = __shfl(ldn, tl, 64);
+            const int tKup = __shfl(Kup, tl, 64), tKdn = __shfl(Kdn, tl, 64);
+            const T twu = __shfl(wu, tl, 64), twd = __shfl(wd, tl, 64);
+            const int tdz = __shfl(dz, tl, 64);
+            ++n_acc;
+            ++acc_since;
+            append_terms(Wup, Wdn, sAu, sBu, sAd, sBd, ns, nsp, nmax, nup, ndn, tKup, tlup, tKdn,
+                         tldn, c, twu, twd);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (tcan1) {
+                    s_kup[ti] = 0; s_kup[tsite] = tlup + 1;
+                    s_kdn[ti] = tldn + 1; s_kdn[tsite] = 0;
+                } else {
+                    s_kup[ti] = tlup + 1; s_kup[tsite] = 0;
+                    s_kdn[ti] = 0; s_kdn[tsite] = tldn + 1;
+                }
+            }
+            ++c;
+            if (c == delay) {
+                stream_update<T, V, DMAX>(Wup, sAu, sBu, ns, nsp, nmax, nup, c);
+                stream_update<T, V, DMAX>(Wdn, sAd, sBd, ns, nsp, nmax, ndn, c);
+                c = 0;
+                ++n_flush;
+            }
+            __syncthreads();
+            zmu += tdz;
+            refresh_due = a.policy == POLICY_REFERENCE ? (t_sw % a.nocc) == 0
+                                                       : acc_since >= a.refresh_k;
+        } else {
+            refresh_due = true;   // rejected move on a reference-cadence refresh sweep
         }
+        if (refresh_due) pause = true;
     }
     __syncthreads();
     for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
@@ -333,10 +395,10 @@ bool sweep_resident(int prec, int ns, int nup, int ndn) {
     return (size_t)ns * (nup + ndn) * es <= kResidentMaxBytes;
 }
 
-size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay) {
+size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay, int nb) {
     const int nmax = nup > ndn ? nup : ndn;
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
-    return sweep_lds_base(es, ns, nmax, delay) +
+    return sweep_lds_base(es, ns, nmax, delay, nb) +
            (sweep_resident(prec, ns, nup, ndn) ? (size_t)ns * (nup + ndn) * es : 0);
 }
 
@@ -393,9 +455,9 @@ __global__ void __launch_bounds__(1024) k_flush(SweepArgs a) {
     const size_t wstride = (size_t)ns * (a.nup + a.ndn);
     T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
     T* Wdn = Wup + (size_t)ns * a.nup;
-    stream_update<T, V, kMaxDelay>(Wup, s_A, s_B, ns, nsp, nmax, a.nup, c);
-    stream_update<T, V, kMaxDelay>(Wdn, s_A + delay * nsp, s_B + delay * nmax, ns, nsp, nmax,
-                                   a.ndn, c);
+    stream_update<T, V, kMaxDelay, 2>(Wup, s_A, s_B, ns, nsp, nmax, a.nup, c);
+    stream_update<T, V, kMaxDelay, 2>(Wdn, s_A + delay * nsp, s_B + delay * nmax, ns, nsp, nmax,
+                                      a.ndn, c);
     __syncthreads();
     if (threadIdx.x == 0) a.npend[w] = 0;
 }

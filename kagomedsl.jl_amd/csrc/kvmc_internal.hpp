@@ -126,7 +126,7 @@ struct ElocArgs {
 // Launchers (defined in kvmc_device.hip).  `prec` 0 = FP64, 1 = FP32.
 hipError_t launch_sweep(int prec, const SweepArgs& a, int nwalkers, int threads, size_t lds,
                         hipStream_t s);
-size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay);
+size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay, int nb);
 // Both W of a walker at most this large: the sweep kernel keeps them in LDS.
 constexpr size_t kResidentMaxBytes = 64 * 1024;
 bool sweep_resident(int prec, int ns, int nup, int ndn);
