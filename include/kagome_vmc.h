@@ -158,6 +158,14 @@ int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out);
 int kvmc_set_sweep_count(kvmc_handle* h, int64_t sweeps);
 int kvmc_get_sweep_count(kvmc_handle* h, int64_t* sweeps);
 
+/* EVERY_K policy only (no effect otherwise): sets each walker's count of
+ * accepted moves since its last refresh to a seeded uniform draw in
+ * [0, refresh_k), so that refreshes are spread over time as in a long run
+ * instead of arriving together after a common start.  Decisions of the
+ * chains are unchanged; only when each W is rebuilt moves.  No reference
+ * counterpart. */
+int kvmc_stagger_refresh(kvmc_handle* h, uint64_t seed);
+
 /* Largest max|W - W_refreshed| / max|W_refreshed| over walkers, measured by
  * recomputing W in FP64 without replacing it (drift of the rank-1 updates). */
 int kvmc_w_drift(kvmc_handle* h, double* max_rel_drift);

@@ -698,6 +698,21 @@ int kvmc_set_sweep_count(kvmc_handle* h, int64_t sweeps) {
     return KVMC_OK;
 }
 
+int kvmc_stagger_refresh(kvmc_handle* h, uint64_t seed) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    if (h->policy != KVMC_REFRESH_EVERY_K) return KVMC_OK;
+    std::vector<int> since(h->nw);
+    for (int w = 0; w < h->nw; ++w) {
+        uint32_t x[4];
+        philox4x32_10((uint32_t)w, 0u, 0u, 2u, (uint32_t)seed, (uint32_t)(seed >> 32), x);
+        since[w] = (int)(((uint64_t)x[0] * (uint64_t)(uint32_t)h->refresh_k) >> 32);
+    }
+    HIP_TRY(hipMemcpyAsync(h->acc_since, since.data(), sizeof(int) * h->nw, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return KVMC_OK;
+}
+
 int kvmc_get_sweep_count(kvmc_handle* h, int64_t* sweeps) {
     g_err.clear();
     if (!h || !sweeps) return fail(KVMC_ERR_INVALID, "bad argument");

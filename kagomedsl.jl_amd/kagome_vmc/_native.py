@@ -24,7 +24,7 @@ EXPORTS = [
     "kvmc_refresh", "kvmc_sweep", "kvmc_measure", "kvmc_set_orbital_derivs",
     "kvmc_measure_grad", "kvmc_get_logdet", "kvmc_get_w", "kvmc_set_sweep_count",
     "kvmc_get_sweep_count", "kvmc_w_drift", "kvmc_failed_walker", "kvmc_last_error",
-    "kvmc_version", "kvmc_profile_enable", "kvmc_get_profile",
+    "kvmc_version", "kvmc_profile_enable", "kvmc_get_profile", "kvmc_stagger_refresh",
 ]
 
 
@@ -100,6 +100,7 @@ def load(path: str = LIB_PATH):
         "kvmc_version": ([], C.c_char_p),
         "kvmc_profile_enable": ([vp, C.c_int32], C.c_int),
         "kvmc_get_profile": ([vp, P(KvmcProfile)], C.c_int),
+        "kvmc_stagger_refresh": ([vp, C.c_uint64], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -237,6 +238,10 @@ class DeviceSampler:
         out = np.zeros(self.ns * n, dtype=np.float64)
         _check(load().kvmc_get_w(self._h, int(walker), int(spin), _ptr(out)))
         return out.reshape((self.ns, n), order="F")
+
+    def stagger_refresh(self, seed):
+        """Spread the EVERY_K refresh phases of the walkers (see the C header)."""
+        _check(load().kvmc_stagger_refresh(self._h, int(seed)))
 
     def set_sweep_count(self, sweeps):
         _check(load().kvmc_set_sweep_count(self._h, int(sweeps)))

@@ -282,3 +282,24 @@ def test_w_drift_is_small_after_many_updates(oracle_mod):
         d64 = ds.w_drift()
     assert d64 < 1e-8
     assert d32 < 1e-2
+
+
+def test_stagger_refresh_moves_refreshes_not_decisions(oracle_mod):
+    """kvmc_stagger_refresh only shifts when walkers refresh: the chains'
+    configurations after a run equal those of an unstaggered run (FP64 W,
+    where a refresh changes W only by rounding)."""
+    ham = system(4)
+    nw = 64
+    ku, kd = well_conditioned_configs(oracle_mod, 21, nw, ham)
+    out = []
+    for stagger in (False, True):
+        with sampler(ham, nw, refresh_policy=1, refresh_k=16, seed=5) as ds:
+            ds.set_configs(ku, kd)
+            if stagger:
+                ds.stagger_refresh(9)
+            cnt, _ = ds.sweep(60)
+            out.append((ds.get_configs(), cnt["refreshes"], cnt["accepted"]))
+    (c0, r0, a0), (c1, r1, a1) = out
+    assert np.array_equal(c0[0], c1[0]) and np.array_equal(c0[1], c1[1])
+    assert a0 == a1
+    assert r1 > r0      # staggered walkers reach their first refresh sooner
