@@ -149,19 +149,23 @@ __device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T*
 // one electron, so each bond touching exactly one of i and j flips its
 // antiparallel status, while bonds between i and j and on-site bonds keep
 // theirs.  dZ = (flipping bonds) - 2 (those antiparallel before).
-__device__ __forceinline__ int z_delta(const int2* bonds, const int* __restrict__ nbr_off,
-                                       const int* __restrict__ nbr, const int* s_kup,
-                                       const int* s_kdn, int i, int j) {
+// Bond tables are LDS-resident as 16-bit site and bond ids: bond b joins
+// sites bonds[b] & 0xffff and bonds[b] >> 16; nbr[nbr_off[x] .. nbr_off[x+1])
+// lists the bonds touching site x.
+__device__ __forceinline__ int z_delta(const uint32_t* bonds, const uint16_t* nbr_off,
+                                       const uint16_t* nbr, const int* s_kup, const int* s_kdn,
+                                       int i, int j) {
     int flip = 0, ap = 0;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const int x = side ? j : i, other = side ? i : j;
         for (int p = nbr_off[x]; p < nbr_off[x + 1]; ++p) {
-            const int2 b = bonds[nbr[p]];
-            const int y = b.x == x ? b.y : b.x;
-            if (b.x == b.y || y == other) continue;
+            const uint32_t bp = bonds[nbr[p]];
+            const int bx = (int)(bp & 0xffffu), by = (int)(bp >> 16);
+            const int y = bx == x ? by : bx;
+            if (bx == by || y == other) continue;
             ++flip;
-            ap += (s_kup[b.x] != 0 && s_kdn[b.y] != 0) || (s_kup[b.y] != 0 && s_kdn[b.x] != 0);
+            ap += (s_kup[bx] != 0 && s_kdn[by] != 0) || (s_kup[by] != 0 && s_kdn[bx] != 0);
         }
     }
     return flip - 2 * ap;
@@ -188,7 +192,7 @@ __device__ inline double block_sum_double(double v, double* red) {
 __host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay, int nb) {
     const size_t nsp = (size_t)(ns + 3) / 4 * 4;
     const size_t b = 2 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
-                     (size_t)nb * sizeof(int2);
+                     (size_t)nb * sizeof(uint32_t) + (size_t)(ns + 1 + 2 * nb) * sizeof(uint16_t);
     return (b + 15) / 16 * 16;
 }
 
@@ -204,7 +208,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     T* s_A = reinterpret_cast<T*>(s_kdn + nsp);           // [2][delay][nsp]
     T* s_B = s_A + 2 * delay * nsp;                       // [2][delay][nmax]
     double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
-    int2* s_bond = reinterpret_cast<int2*>(s_red + 16);   // [nb] bond table
+    uint32_t* s_bond = reinterpret_cast<uint32_t*>(s_red + 16);   // [nb] packed bond table
+    uint16_t* s_noff = reinterpret_cast<uint16_t*>(s_bond + nb);    // [ns + 1]
+    uint16_t* s_nbr = s_noff + ns + 1;                              // [<= 2 nb]
 
     const size_t wstride = (size_t)ns * (nup + ndn);
     T* const Wg = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
@@ -235,7 +241,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     if (RES) {
         for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wup[t] = Wg[t];
     }
-    for (int b = (int)threadIdx.x; b < nb; b += (int)blockDim.x) s_bond[b] = a.bonds[b];
+    for (int b = (int)threadIdx.x; b < nb; b += (int)blockDim.x) {
+        const int2 bd = a.bonds[b];
+        s_bond[b] = (uint32_t)bd.x | ((uint32_t)bd.y << 16);
+    }
+    for (int t = (int)threadIdx.x; t <= ns; t += (int)blockDim.x) s_noff[t] = (uint16_t)a.nbr_off[t];
+    for (int t = (int)threadIdx.x; t < a.nbr_off[ns]; t += (int)blockDim.x) s_nbr[t] = (uint16_t)a.nbr[t];
     __syncthreads();
     // Z_mu: full recount once per launch, then maintained incrementally.
     int zmu;
@@ -283,9 +294,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         bool can1 = false;
         T wu = T(0), wd = T(0);
         if (valid && !(a.kernel_mode == KERNEL_REFERENCE && r > zr)) {
-            const int2 bd = s_bond[bidx];
-            i = bd.x;
-            site = bd.y;
+            const uint32_t bp = s_bond[bidx];
+            i = (int)(bp & 0xffffu);
+            site = (int)(bp >> 16);
             const int iu = s_kup[i], su = s_kup[site], id = s_kdn[i], sd = s_kdn[site];
             can1 = iu != 0 && sd != 0;
             const bool can2 = su != 0 && id != 0;
@@ -300,7 +311,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
                 Kdn = can1 ? i : site;
                 wu = w_cur(Wup, sAu, sBu, ns, nsp, nmax, Kup, lup, c);
                 wd = w_cur(Wdn, sAd, sBd, ns, nsp, nmax, Kdn, ldn, c);
-                dz = z_delta(s_bond, a.nbr_off, a.nbr, s_kup, s_kdn, i, site);
+                dz = z_delta(s_bond, s_noff, s_nbr, s_kup, s_kdn, i, site);
                 const T ratio = wu * wd;
                 const double acc_prob = (double)ratio * (double)ratio;
                 if (a.kernel_mode == KERNEL_REFERENCE)
