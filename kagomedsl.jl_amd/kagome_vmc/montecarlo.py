@@ -56,8 +56,9 @@ class MC:
     link_in, link_inter`` (optional).  Device keys: ``nwalkers`` (1),
     ``precision`` ("fp64" | "fp32"), ``kernel`` ("reference" | "unbiased"),
     ``refresh`` ("reference" or an int k: refresh every k accepted moves),
-    ``delay`` (pending rank-1 updates per flush, fp32 only), ``threads``,
-    ``seed`` (device RNG key; defaults to the context seed)."""
+    ``delay`` (pending rank-1 updates per flush, fp32 only), ``threads``
+    (either may be "auto": chosen by the autotuner), ``seed`` (device RNG
+    key; defaults to the context seed)."""
 
     def __init__(self, params: dict):
         n1, n2 = int(params["n1"]), int(params["n2"])
@@ -74,12 +75,19 @@ class MC:
         refresh = params.get("refresh", "reference")
         policy = nat.REFRESH_REFERENCE if refresh == "reference" else nat.REFRESH_EVERY_K
         k = 0 if refresh == "reference" else int(refresh)
+        prec = _PREC[params.get("precision", "fp64")]
+        threads, delay = params.get("threads", 0), params.get("delay", 1)
+        if threads == "auto" or delay == "auto":
+            from .autotune import autotune
+            tuned = autotune(self.Ham, self.nwalkers, prec, k or min(self.N_up, self.N_down),
+                             seed=int(params.get("seed", 0)))
+            threads = tuned["threads"] if threads == "auto" else threads
+            delay = tuned["delay"] if delay == "auto" else delay
         self.sampler = nat.DeviceSampler(
             self.Ham.bonds_array(), self.Ham.U_up, self.Ham.U_down, self.nwalkers,
-            precision=_PREC[params.get("precision", "fp64")],
-            kernel=_KERNEL[params.get("kernel", "reference")],
-            refresh_policy=policy, refresh_k=k, delay=int(params.get("delay", 1)),
-            threads=int(params.get("threads", 0)), seed=int(params.get("rng_seed", params.get("seed", 0))))
+            precision=prec, kernel=_KERNEL[params.get("kernel", "reference")],
+            refresh_policy=policy, refresh_k=k, delay=int(delay), threads=int(threads),
+            seed=int(params.get("rng_seed", params.get("seed", 0))))
         self.counters = {"sweeps": 0, "moves": 0, "accepted": 0, "refreshes": 0}
 
     @property

@@ -30,3 +30,11 @@ def test_groups_match_their_solo_runs():
         for a, b in zip(res[g][1], e_solo):
             assert np.array_equal(a, b)
         assert res[g][0]["sweeps"] == 3 * nw * ham.N_up
+
+
+def test_autotune_on_device(tmp_path):
+    from kagome_vmc.autotune import autotune, candidates
+    ham = pi_flux_cluster(4)
+    r = autotune(ham, 128, nat.PREC_FP32, 32, steps=1, cache_path=str(tmp_path / "t.json"))
+    assert (r["threads"], r["delay"]) in candidates(ham.ns, nat.PREC_FP32)
+    assert r["walker_sweeps_per_s"] > 0 and len(r["trials"]) == len(candidates(ham.ns, nat.PREC_FP32))
