@@ -104,13 +104,16 @@ def run_form(nat, ham, args, precision, delay, k, label):
     # refresh phases spread as in a long run (every-k policy), so that any
     # timed window holds the steady-state refresh rate
     ds.stagger_refresh(args.seed)
+    if args.graph:
+        ds.enable_graph(True)
     for w in range(args.warmup):
         t0 = time.perf_counter()
         ds.sweep(S)
         ds.measure()
         log(f"[{label}] warmup step {w}: {time.perf_counter() - t0:.3f}s")
     sync()
-    ds.profile(True)
+    # graph replay needs profiling off: kernel times are then not split
+    ds.profile(not args.graph)
     tot = dict(sweeps=0, moves=0, accepted=0, refreshes=0, flushes=0, launches=0)
     nap = 0
     energies = []
@@ -149,19 +152,23 @@ def run_form(nat, ham, args, precision, delay, k, label):
     b_eloc = s * 2 * nap + 8 * ns * nw * args.steps
     flops_walker = refresh_flops(ns, n) + refresh_flops(ns, ham.N_down)
     flops_ref = tot["refreshes"] * flops_walker
+    def rate(num, ms):
+        return num / (ms * 1e-3) if ms > 0 else None
+
     res = {
         "walker_sweeps_per_s": nw * S * args.steps / wall,
         "ms_per_step": 1e3 * wall / args.steps,
-        "eloc_evals_per_s_kernel": nw * args.steps / (prof["measure_ms"] * 1e-3),
+        "graph_replay": bool(args.graph),
+        "eloc_evals_per_s_kernel": rate(nw * args.steps, prof["measure_ms"]),
         "accept_rate": tot["accepted"] / tot["sweeps"],
         "counters": tot,
         "kernel_ms": {k2: prof[k2] for k2 in ("sweep_ms", "refresh_ms", "measure_ms")},
         "launches": {k2: prof[k2] for k2 in ("sweep_launches", "refresh_launches", "measure_launches")},
         "sweep_bytes": b_sweep,
-        "sweep_gbs": b_sweep / (prof["sweep_ms"] * 1e-3) / 1e9,
-        "rank1_equiv_gbs": tot["accepted"] * 2 * s * ns * (n + ham.N_down) / (prof["sweep_ms"] * 1e-3) / 1e9,
-        "eloc_gbs": b_eloc / (prof["measure_ms"] * 1e-3) / 1e9,
-        "refresh_tflops": (flops_ref / (prof["refresh_ms"] * 1e-3) / 1e12) if prof["refresh_ms"] > 0 else None,
+        "sweep_gbs": (rate(b_sweep, prof["sweep_ms"]) or 0) / 1e9,
+        "rank1_equiv_gbs": (rate(tot["accepted"] * 2 * s * ns * (n + ham.N_down), prof["sweep_ms"]) or 0) / 1e9,
+        "eloc_gbs": (rate(b_eloc, prof["measure_ms"]) or 0) / 1e9,
+        "refresh_tflops": (rate(flops_ref, prof["refresh_ms"]) or 0) / 1e12 or None,
         "refresh_flops_per_walker": flops_walker,
         "refresh_walkers_per_launch": (tot["refreshes"] / prof["refresh_launches"]) if prof["refresh_launches"] else 0,
         "refresh_ms_per_launch": (prof["refresh_ms"] / prof["refresh_launches"]) if prof["refresh_launches"] else 0,
@@ -225,7 +232,7 @@ def copy_calibration(mib):
 
 def roofline_refresh(r, ns, traffic_per_walker):
     wpl = r["refresh_walkers_per_launch"]
-    if not wpl:
+    if not wpl or not r["refresh_ms_per_launch"]:
         return None
     achieved = r["refresh_flops_per_walker"] * wpl / (r["refresh_ms_per_launch"] * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -236,6 +243,8 @@ def roofline_refresh(r, ns, traffic_per_walker):
 
 
 def roofline_sweep(r, traffic_per_walker_sweep):
+    if not r["sweep_ms_per_launch"]:
+        return None
     achieved = r["sweep_bytes_per_launch"] / (r["sweep_ms_per_launch"] * 1e-3) / 1e9
     wsl = r["walker_sweeps_per_launch"]
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -271,11 +280,16 @@ def run_mixed(args):
         return time.perf_counter() - t0, res
 
     with MixedBatch(groups, **kw) as mb:
+        if args.graph:
+            for g in mb.groups:
+                g.sampler.enable_graph(True)
         wall, res = timed(mb)
     per = []
     alone_total = 0.0
     for (ham, nw), (cnt, _) in zip(groups, res):
         with MixedBatch([(ham, nw)], **kw) as one:
+            if args.graph:
+                one.groups[0].sampler.enable_graph(True)
             t1, _ = timed(one)
         alone_total += t1
         per.append({"L": int(round((ham.ns / 3) ** 0.5)), "walkers": nw, "walker_sweeps": cnt["sweeps"],
@@ -316,6 +330,8 @@ def main():
     ap.add_argument("--drift", action="store_true")
     ap.add_argument("--probe-steps", type=int, default=4,
                     help="untimed steps after the timed region, each checking the carried W")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay captured sweep sequences (no per-kernel times)")
     ap.add_argument("--mixed", action="store_true",
                     help="only run the mixed-cluster-size batch (L = 4..12 groups at once)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -158,6 +158,17 @@ int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out);
 int kvmc_set_sweep_count(kvmc_handle* h, int64_t sweeps);
 int kvmc_get_sweep_count(kvmc_handle* h, int64_t* sweeps);
 
+/* Enables (1) or disables (0) graph replay for kvmc_sweep calls without a
+ * tape, outcome record or profiling.  The first such call for a given
+ * nsweeps captures on the handle's stream: stop counts, counter resets (by
+ * fill kernels), then 3 rounds of the sweep kernel and the refresh of the
+ * walkers it paused, their count read on the device.  Later calls replay
+ * it, and walkers still paused after the last round are finished by the
+ * ordinary launch loop.  Results are identical to the launch loop.  No
+ * reference counterpart: it removes per-launch host round trips for small
+ * clusters. */
+int kvmc_enable_graph(kvmc_handle* h, int32_t enable);
+
 /* EVERY_K policy only (no effect otherwise): sets each walker's count of
  * accepted moves since its last refresh to a seeded uniform draw in
  * [0, refresh_k), so that refreshes are spread over time as in a long run

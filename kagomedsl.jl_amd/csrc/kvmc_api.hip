@@ -97,6 +97,16 @@ __global__ void k_fill_i(int* p, int v, int n) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_ull(unsigned long long* p, unsigned long long v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Sweep launches per replay of a captured sweep sequence: sweep, refresh
+// the walkers that paused, sweep again, ...; walkers still paused after the
+// last round are finished by the host loop.
+constexpr int kGraphRounds = 3;
+
 }  // namespace
 
 struct kvmc_handle {
@@ -145,6 +155,10 @@ struct kvmc_handle {
     int64_t failed_walker = -1;
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // captured sweep sequence (kvmc_enable_graph)
+    bool graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int64_t g_nsweeps = -1;
     kvmc_profile prof{};
 };
 
@@ -159,6 +173,7 @@ void free_all(kvmc_handle* h) {
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -194,11 +209,7 @@ SweepArgs sweep_args(kvmc_handle* h) {
 
 // Refreshes the walkers listed in d_list[0..count) (device array).  When
 // write_w is 0 only the drift against the current W is measured.
-int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
-                 double pivot_tol = 0.0) {
-    if (count <= 0) return KVMC_OK;
-    static const int kNone = INT_MAX;
-    HIP_TRY(hipMemcpyAsync(h->failed, &kNone, sizeof(int), hipMemcpyHostToDevice, h->stream));
+RefreshArgs refresh_args(const kvmc_handle* h, int write_w, double pivot_tol) {
     RefreshArgs r{};
     r.ns = h->ns; r.nup = h->nup; r.ndn = h->ndn; r.Uup = h->Uup; r.Udn = h->Udn;
     r.UTup = h->UTup; r.UTdn = h->UTdn;
@@ -206,6 +217,15 @@ int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
     r.npend = h->npend; r.acc_since = h->acc_since; r.delay = h->delay; r.nmax = h->nmax;
     r.ws = h->ws; r.logdet = h->logdet; r.status = h->status; r.failed = h->failed;
     r.write_w = write_w; r.drift = h->drift; r.pivot_tol = pivot_tol;
+    return r;
+}
+
+int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
+                 double pivot_tol = 0.0) {
+    if (count <= 0) return KVMC_OK;
+    static const int kNone = INT_MAX;
+    HIP_TRY(hipMemcpyAsync(h->failed, &kNone, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    RefreshArgs r = refresh_args(h, write_w, pivot_tol);
     const int batch_walkers = std::max(1, h->ws_jobs / 2);
     prof_begin(h);
     for (int b0 = 0; b0 < count; b0 += batch_walkers) {
@@ -528,15 +548,74 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
     const int threads = sweep_threads(h);
     const size_t lds = sweep_lds_bytes(h->prec, h->ns, h->nup, h->ndn, h->delay, h->nb);
     if (lds > 160 * 1024) { cleanup(); return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay"); }
-    int64_t launches = 0, refreshes = 0;
+    int64_t launches = 0;
     hipStream_t s = h->stream;
-    {
+    const bool use_graph = h->graph && !tape_u && !outcomes && !h->profiling;
+    bool resume = false;   // walkers are paused and await a refresh
+    if (use_graph) {
+        // One replay: stop counts, counter and flag resets (fill kernels, no
+        // memset nodes), then kGraphRounds rounds of sweep + refresh of the
+        // paused walkers with the count read on the device.
+        if (!h->gexec || h->g_nsweeps != nsweeps) {
+            if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            const int fb = (h->nw + 255) / 256;
+            hipLaunchKernelGGL(k_set_stop, dim3(fb), dim3(256), 0, s, h->sweeps, h->stop, h->tbase,
+                               (long long)nsweeps, h->nw);
+            hipLaunchKernelGGL(k_fill_ull, dim3(1), dim3(64), 0, s, h->counters, 0ull, (int)CNT_SLOTS);
+            hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->failed, INT_MAX, 1);
+            hipError_t ce = hipSuccess;
+            for (int r = 0; r < kGraphRounds && ce == hipSuccess; ++r) {
+                hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->rcount, 0, 1);
+                ce = launch_sweep(h->prec, a, h->nw, threads, lds, s);
+                if (r + 1 == kGraphRounds || ce != hipSuccess) break;
+                RefreshArgs ra = refresh_args(h, 1, 0.0);
+                const int batch = std::max(1, h->ws_jobs / 2);
+                for (int b0 = 0; b0 < h->nw && ce == hipSuccess; b0 += batch) {
+                    ra.jobs = h->rlist + b0;
+                    ra.njobs = 2 * std::min(batch, h->nw - b0);
+                    ra.count_dev = h->rcount;
+                    ra.walker_off = b0;
+                    ce = launch_refresh(h->prec, ra, s);
+                }
+            }
+            hipError_t ee = hipStreamEndCapture(s, &graph);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                cleanup();
+                return fail(KVMC_ERR_HIP, std::string("sweep graph capture: ") +
+                                              hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) { h->gexec = nullptr; cleanup(); return fail(KVMC_ERR_HIP, "sweep graph instantiate failed"); }
+            h->g_nsweeps = nsweeps;
+        }
+        HIP_TRY(hipGraphLaunch(h->gexec, s));
+        launches += kGraphRounds;
+        int count = 0, failed = INT_MAX;
+        HIP_TRY(hipMemcpyAsync(&count, h->rcount, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&failed, h->failed, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (failed != INT_MAX) {
+            h->failed_walker = failed;
+            cleanup();
+            return fail(KVMC_ERR_SINGULAR, "tildeU is singular for walker " + std::to_string(failed) +
+                                               " (lu factorization failed)");
+        }
+        if (count == 0) goto done;
+        rc = refresh_list(h, h->rlist, count, 1);
+        if (rc) { cleanup(); return rc; }
+        resume = true;
+    } else {
         hipLaunchKernelGGL(k_set_stop, dim3((h->nw + 255) / 256), dim3(256), 0, s,
                            h->sweeps, h->stop, h->tbase, (long long)nsweeps, h->nw);
         if (hipMemsetAsync(h->counters, 0, sizeof(unsigned long long) * CNT_SLOTS, s) != hipSuccess) {
             cleanup(); return fail(KVMC_ERR_HIP, "counter reset failed");
         }
     }
+    (void)resume;
     for (;;) {
         int count = 0;
         if (hipMemsetAsync(h->rcount, 0, sizeof(int), s) != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, "hipMemsetAsync failed"); }
@@ -553,10 +632,10 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         h->prof.sweep_ms += prof_elapsed(h);
         h->prof.sweep_launches += h->profiling;
         if (count == 0) break;
-        refreshes += count;
         rc = refresh_list(h, h->rlist, count, 1);
         if (rc) { cleanup(); return rc; }
     }
+done:
     if (outcomes &&
         hipMemcpyAsync(outcomes, d_out, tape_n, hipMemcpyDeviceToHost, s) != hipSuccess) {
         cleanup(); return fail(KVMC_ERR_HIP, "outcome copy failed");
@@ -571,7 +650,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         counters->sweeps = (int64_t)cnt[CNT_SWEEPS];
         counters->moves = (int64_t)cnt[CNT_MOVES];
         counters->accepted = (int64_t)cnt[CNT_ACCEPTED];
-        counters->refreshes = refreshes;
+        counters->refreshes = (int64_t)cnt[CNT_PAUSES];
         counters->flushes = (int64_t)cnt[CNT_FLUSHES];
         counters->launches = launches;
     }
@@ -697,6 +776,18 @@ int kvmc_set_sweep_count(kvmc_handle* h, int64_t sweeps) {
                        (long long)sweeps, h->nw);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return KVMC_OK;
+}
+
+int kvmc_enable_graph(kvmc_handle* h, int32_t enable) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    h->graph = enable != 0;
+    if (!h->graph && h->gexec) {
+        HIP_TRY(hipGraphExecDestroy(h->gexec));
+        h->gexec = nullptr;
+        h->g_nsweeps = -1;
+    }
     return KVMC_OK;
 }
 

@@ -393,6 +393,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         if (pause) {
             int slot = atomicAdd(a.refresh_count, 1);
             a.refresh_list[slot] = w;
+            atomicAdd(&a.counters[CNT_PAUSES], 1ull);
         }
         atomicAdd(&a.counters[CNT_MOVES], n_moves);
         atomicAdd(&a.counters[CNT_ACCEPTED], n_acc);

@@ -14,7 +14,7 @@ enum : int { POLICY_REFERENCE = 0, POLICY_EVERY_K = 1 };
 
 // Device-wide counters (u64 slots).
 enum : int { CNT_MOVES = 0, CNT_ACCEPTED = 1, CNT_FLUSHES = 2, CNT_SWEEPS = 3, CNT_NAP = 4,
-             CNT_SLOTS = 8 };
+             CNT_PAUSES = 5, CNT_SLOTS = 8 };
 
 // Largest number of pending rank-1 updates kept per spin before a flush.
 constexpr int kMaxDelay = 16;
@@ -94,6 +94,8 @@ struct RefreshArgs {
     int delay, nmax;
     const int* jobs;                // walker ids; job j -> walker jobs[j/2], spin j%2
     int njobs;
+    const int* count_dev;           // when set: only the first 2 * (*count_dev - walker_off) jobs run
+    int walker_off;
     double* ws;                     // [njobs][refresh_ws_doubles_per_job] FP64 work matrices
     double* logdet;                 // [nw][2]
     int* status;                    // [nw] 0 ok, 1 singular

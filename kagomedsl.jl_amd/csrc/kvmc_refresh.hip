@@ -161,6 +161,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     extern __shared__ __align__(16) unsigned char lds[];
     const int job = (int)blockIdx.x;
     if (job >= a.njobs) return;
+    if (a.count_dev && job >= 2 * (*a.count_dev - a.walker_off)) return;
     const int w = a.jobs[job >> 1];
     const int spin = job & 1;
     const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;

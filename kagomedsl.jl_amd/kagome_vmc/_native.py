@@ -25,6 +25,7 @@ EXPORTS = [
     "kvmc_measure_grad", "kvmc_get_logdet", "kvmc_get_w", "kvmc_set_sweep_count",
     "kvmc_get_sweep_count", "kvmc_w_drift", "kvmc_failed_walker", "kvmc_last_error",
     "kvmc_version", "kvmc_profile_enable", "kvmc_get_profile", "kvmc_stagger_refresh",
+    "kvmc_enable_graph",
 ]
 
 
@@ -101,6 +102,7 @@ def load(path: str = LIB_PATH):
         "kvmc_profile_enable": ([vp, C.c_int32], C.c_int),
         "kvmc_get_profile": ([vp, P(KvmcProfile)], C.c_int),
         "kvmc_stagger_refresh": ([vp, C.c_uint64], C.c_int),
+        "kvmc_enable_graph": ([vp, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -238,6 +240,10 @@ class DeviceSampler:
         out = np.zeros(self.ns * n, dtype=np.float64)
         _check(load().kvmc_get_w(self._h, int(walker), int(spin), _ptr(out)))
         return out.reshape((self.ns, n), order="F")
+
+    def enable_graph(self, enable=True):
+        """Replay a captured sweep sequence instead of launching per round."""
+        _check(load().kvmc_enable_graph(self._h, int(bool(enable))))
 
     def stagger_refresh(self, seed):
         """Spread the EVERY_K refresh phases of the walkers (see the C header)."""

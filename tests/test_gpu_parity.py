@@ -303,3 +303,31 @@ def test_stagger_refresh_moves_refreshes_not_decisions(oracle_mod):
     assert np.array_equal(c0[0], c1[0]) and np.array_equal(c0[1], c1[1])
     assert a0 == a1
     assert r1 > r0      # staggered walkers reach their first refresh sooner
+
+
+@pytest.mark.parametrize("L,prec,policy,k", [(4, 1, 1, 8), (4, 0, 0, 0), (8, 1, 1, 16), (8, 0, 1, 4)])
+def test_graph_replay_matches_launch_loop(oracle_mod, L, prec, policy, k):
+    """kvmc_enable_graph: every replay (with eager measurements and a change
+    of nsweeps between replays) leaves the same configurations, energies and
+    counters as the ordinary launch loop."""
+    ham = system(L)
+    nw = 96
+    ku, kd = well_conditioned_configs(oracle_mod, 31, nw, ham)
+    runs = []
+    for graph in (False, True):
+        with sampler(ham, nw, precision=prec, refresh_policy=policy, refresh_k=k, delay=4 if prec else 1,
+                     seed=13) as ds:
+            ds.set_configs(ku, kd)
+            if graph:
+                ds.enable_graph(True)
+            rec = []
+            for n in (ham.N_up, ham.N_up, 7, ham.N_up):
+                cnt, _ = ds.sweep(n)
+                e, _ = ds.measure()
+                rec.append((cnt, e, ds.get_configs()))
+            runs.append(rec)
+    for (c0, e0, k0), (c1, e1, k1) in zip(*runs):
+        for key in ("sweeps", "moves", "accepted", "refreshes", "flushes"):
+            assert c0[key] == c1[key], key
+        assert np.array_equal(e0, e1)
+        assert np.array_equal(k0[0], k1[0]) and np.array_equal(k0[1], k1[1])
