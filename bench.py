@@ -308,6 +308,56 @@ def run_mixed(args):
     print(json.dumps(line), flush=True)
 
 
+def run_grad(args):
+    """Gradient throughput: walkers from |psi|^2 (unbiased kernel, FP64 W) on
+    the ansatz family at theta0, one step = N sweeps + one local-energy and
+    log-derivative pass; reports measure_grad evaluations/s and the sampled
+    gradient with its jackknife-free standard error over walkers."""
+    from kagome_vmc import _native as nat
+    from kagome_vmc.gradient import THETA0, GradientSampler, ansatz_family
+    from kagome_vmc.lattice import DoubleKagome
+    nat.load()
+    L = args.L
+    pbc, apbc = BC[L]
+    fam = ansatz_family(DoubleKagome.build(1.0, L, L, pbc, apbc))
+    n = 3 * L * L // 2
+    gs = GradientSampler(fam, THETA0, n, n, args.walkers, seed=args.seed, precision=nat.PREC_FP64)
+    s = gs.sampler
+    s.init_random(args.seed)
+    for _ in range(args.warmup):
+        s.sweep(n)
+    sync()
+    t_grad = 0.0
+    E, O = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.sweep(n)
+        t1 = time.perf_counter()
+        e, o = s.measure_grad()
+        t_grad += time.perf_counter() - t1
+        E.append(e)
+        O.append(o)
+    wall = time.perf_counter() - t0
+    gs.close()
+    E, O = np.stack(E), np.stack(O)
+    w_e = E.mean(0)
+    w_g = 2.0 * ((E[..., None] * O).mean(0) - E.mean(0)[:, None] * O.mean(0))
+    line = {
+        "metric": f"gradient evaluations/s (E_L and 4 log-derivatives per walker), L={L}",
+        "value": args.walkers * args.steps / t_grad, "unit": "walker-evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"pi-flux Kagome L={L}, {args.walkers} walkers from |psi|^2, "
+                               f"theta0 = (1, 0, 0, 0); step = {n} sweeps + measure_grad"},
+        "energy_per_site": float(w_e.mean() / (3 * L * L)),
+        "gradient_per_site": (w_g.mean(0) / (3 * L * L)).tolist(),
+        "gradient_stderr_per_site": (w_g.std(0, ddof=1) / np.sqrt(args.walkers) / (3 * L * L)).tolist(),
+        "walker_sweeps_per_s": args.walkers * n * args.steps / wall,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +382,8 @@ def main():
                     help="untimed steps after the timed region, each checking the carried W")
     ap.add_argument("--graph", action="store_true",
                     help="replay captured sweep sequences (no per-kernel times)")
+    ap.add_argument("--grad", action="store_true",
+                    help="only run the gradient workload (unbiased kernel, FP64, log-derivatives)")
     ap.add_argument("--mixed", action="store_true",
                     help="only run the mixed-cluster-size batch (L = 4..12 groups at once)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -347,6 +399,9 @@ def main():
         return
     if args.mixed:
         run_mixed(args)
+        return
+    if args.grad:
+        run_grad(args)
         return
 
     from kagome_vmc import DoubleKagome, Hamiltonian
