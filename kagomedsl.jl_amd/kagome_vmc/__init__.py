@@ -9,8 +9,8 @@ libkagome_vmc.so on the GPU.
 from .lattice import (DoubleKagome, get_boundary_shifts, get_site_coord, ns, unitcell_coord,
                       unitcell_diff)
 from .hamiltonian import Hamiltonian, Hmat, get_nn, orbitals, pi_link_in, pi_link_inter, shell_gap
-from .montecarlo import (MC, MCContext, init, measure, read_checkpoint, register_evaluables, run,
-                         sweep, write_checkpoint)
+from .montecarlo import (MC, MCContext, find_initial_configuration, init, init_conf_qr, measure,
+                         read_checkpoint, register_evaluables, run, sweep, write_checkpoint)
 from .gradient import (AnsatzFamily, GradientSampler, ansatz_family, energy, exhaustive_average,
                        orbital_derivatives)
 from . import _native as native
@@ -20,5 +20,6 @@ __all__ = [
     "get_boundary_shifts", "Hamiltonian", "Hmat", "get_nn", "orbitals", "pi_link_in",
     "pi_link_inter", "shell_gap", "MC", "MCContext", "init", "sweep", "measure", "run",
     "register_evaluables", "write_checkpoint", "read_checkpoint", "native", "AnsatzFamily",
+    "init_conf_qr", "find_initial_configuration",
     "GradientSampler", "ansatz_family", "energy", "exhaustive_average", "orbital_derivatives",
 ]

@@ -103,15 +103,56 @@ class MC:
         self.sampler.close()
 
 
+def init_conf_qr(U_up: np.ndarray, U_down: np.ndarray):
+    """init_conf_qr! (src/MonteCarlo.jl:326-357): up electrons on the first
+    N_up pivot columns of a column-pivoted QR of U_up^T; down electrons on the
+    first N_down pivots of the same factorisation of U_down^T restricted to
+    the remaining sites (in increasing site order, as setdiff(1:ns, ...)).
+    Labels follow the pivot order.  Returns (kappa_up, kappa_down), 1-based."""
+    import scipy.linalg as sla
+    U_up = np.real_if_close(np.asarray(U_up))
+    U_down = np.real_if_close(np.asarray(U_down))
+    ns, n_up = U_up.shape
+    n_dn = U_down.shape[1]
+    kup = np.zeros(ns, dtype=np.int32)
+    kdn = np.zeros(ns, dtype=np.int32)
+    _, _, p_up = sla.qr(U_up.T, mode="economic", pivoting=True)
+    sites_up = p_up[:n_up]
+    kup[sites_up] = np.arange(1, n_up + 1)
+    if n_dn > 0:
+        avail = np.setdiff1d(np.arange(ns), sites_up)
+        _, _, p_dn = sla.qr(U_down[avail, :].T, mode="economic", pivoting=True)
+        kdn[avail[p_dn[:n_dn]]] = np.arange(1, n_dn + 1)
+    return kup, kdn
+
+
+def find_initial_configuration(sampler, U_up, U_down):
+    """find_initial_configuration! (src/MonteCarlo.jl:382-410): every walker
+    starts from the QR configuration; a singular tildeU raises the
+    reference's error text."""
+    kup, kdn = init_conf_qr(U_up, U_down)
+    nw = sampler.nwalkers
+    try:
+        sampler.set_configs(np.tile(kup, (nw, 1)), np.tile(kdn, (nw, 1)))
+    except nat.SingularError as e:
+        raise RuntimeError("QR-based configuration is singular. The Hamiltonian may be "
+                           "rank-deficient.") from e
+    return kup, kdn
+
+
 def init(mc: MC, ctx: MCContext, params: dict | None = None, kappa=None):
-    """Carlo.init! (src/MonteCarlo.jl:435-441).  The reference starts one chain
-    from a pivoted-QR configuration; the batch starts every walker from a
-    seeded uniform Sz=0 configuration (redrawn while tildeU is singular), or
-    from ``kappa = (kappa_up, kappa_down)`` when given."""
+    """Carlo.init! (src/MonteCarlo.jl:435-441).  With ``params["init"] ==
+    "qr"`` every walker starts from the reference's deterministic pivoted-QR
+    configuration; by default each walker starts from a seeded uniform Sz=0
+    configuration (redrawn while tildeU is singular); ``kappa = (kappa_up,
+    kappa_down)`` sets the configurations directly."""
+    params = params or {}
     if kappa is not None:
         mc.sampler.set_configs(*kappa)
+    elif params.get("init") == "qr":
+        find_initial_configuration(mc.sampler, mc.Ham.U_up, mc.Ham.U_down)
     else:
-        seed = int((params or {}).get("init_seed", ctx.seed))
+        seed = int(params.get("init_seed", ctx.seed))
         mc.sampler.init_random(seed)
     mc.sampler.set_sweep_count(ctx.sweeps)
 

@@ -61,3 +61,31 @@ def test_checkpoint_resume_continues_the_chain(tmp_path):
         assert ctx2.sweeps == 42
     finally:
         mc2.close()
+
+
+def test_qr_start_and_its_singular_error():
+    """Carlo.init! with the reference's QR start (src/MonteCarlo.jl:382-441):
+    a pi-flux cluster starts from a valid, non-singular configuration; a
+    rank-1 U_up raises the reference's error text
+    (test/test-MonteCarlo.jl:429-477)."""
+    from kagome_vmc import _native as nat
+    from kagome_vmc.montecarlo import find_initial_configuration
+    params = dict(PARAMS, nwalkers=8, init="qr")
+    mc = MC(params)
+    try:
+        ctx = MCContext.from_params(params)
+        init(mc, ctx, params)
+        ku, kd = mc.kappa
+        assert np.all((ku != 0) ^ (kd != 0))
+        assert np.all(ku == ku[0])
+        run(mc, ctx, 12)
+    finally:
+        mc.close()
+    rng = np.random.default_rng(0)
+    U_bad = np.repeat(rng.random((1, 6)), 12, axis=0)
+    ds = nat.DeviceSampler(np.array([[1, 2]], dtype=np.int32), U_bad, rng.random((12, 6)), 2)
+    try:
+        with pytest.raises(RuntimeError, match="QR-based configuration is singular"):
+            find_initial_configuration(ds, U_bad, rng.random((12, 6)))
+    finally:
+        ds.close()

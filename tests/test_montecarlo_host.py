@@ -86,3 +86,42 @@ def test_energy_evaluable_normalises_by_sites():
     e, err = register_evaluables(params)["energy"](ctx)
     assert math.isclose(e, -3.0 / 12)
     assert math.isclose(err, np.std([-0.2, -0.3], ddof=1) / math.sqrt(2))
+
+
+# --- init_conf_qr! (src/MonteCarlo.jl:326-357), restating the properties the
+# reference's own test checks (test/test-MonteCarlo.jl:14-52) ----------------
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_init_conf_qr_random_orbitals(seed, oracle_mod):
+    from kagome_vmc.montecarlo import init_conf_qr
+    rng = np.random.default_rng(seed)
+    ns, n_up = 12, 6
+    U_up, U_dn = rng.random((ns, n_up)), rng.random((ns, ns - n_up))
+    ku, kd = init_conf_qr(U_up, U_dn)
+    assert np.count_nonzero(ku) == n_up and np.count_nonzero(kd) == ns - n_up
+    assert sorted(ku[ku != 0]) == list(range(1, n_up + 1))
+    assert sorted(kd[kd != 0]) == list(range(1, ns - n_up + 1))
+    assert np.all((ku != 0) ^ (kd != 0))
+    assert abs(np.linalg.det(oracle_mod.tilde_u(U_up, ku))) > np.finfo(float).eps
+    assert abs(np.linalg.det(oracle_mod.tilde_u(U_dn, kd))) > np.finfo(float).eps
+
+
+@pytest.mark.parametrize("L", [2, 4, 8, 12])
+def test_init_conf_qr_pi_flux_is_well_conditioned(L, oracle_mod):
+    from kagome_vmc.montecarlo import init_conf_qr
+    from kagome_systems import system
+    ham = system(L)
+    ku, kd = init_conf_qr(ham.U_up, ham.U_down)
+    assert np.all((ku != 0) ^ (kd != 0))
+    # column-pivoted QR picks well-spread rows: far from the 1e-10 redraw bar
+    for U, k in ((ham.U_up, ku), (ham.U_down, kd)):
+        assert np.linalg.cond(oracle_mod.tilde_u(U, k).real) < 1e8
+
+
+def test_init_conf_qr_pivot_order_is_lapack_geqp3():
+    """The labels follow the pivot order of LAPACK's column-pivoted QR (the
+    routine behind Julia's qr(A, ColumnNorm())): the first pivot is the
+    column of largest norm."""
+    from kagome_vmc.montecarlo import init_conf_qr
+    U = np.eye(4)[:, :2] * np.array([1.0, 3.0]) + 0.01
+    ku, kd = init_conf_qr(U, np.ones((4, 2)))
+    assert ku[1] == 1          # site 2 (row norm 3) is pivot 1
