@@ -93,7 +93,8 @@ def run_form(nat, ham, args, precision, delay, k, label):
     ns, n = ham.ns, ham.N_up
     nw = args.walkers
     S = args.sweeps_per_step or n
-    policy = nat.REFRESH_EVERY_K if k > 0 else nat.REFRESH_REFERENCE
+    policy = ({"every_k": nat.REFRESH_EVERY_K, "at_end": nat.REFRESH_AT_END}[args.policy]
+              if k > 0 else nat.REFRESH_REFERENCE)
     ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, precision=precision,
                            refresh_policy=policy, refresh_k=max(k, 0), delay=delay,
                            threads=args.threads, seed=args.seed)
@@ -371,6 +372,8 @@ def main():
     ap.add_argument("--k", type=int, default=216, help="FP32 form: accepted moves per refresh")
     ap.add_argument("--k64", type=int, default=0,
                     help="FP64 form: accepted moves per refresh (0: the reference cadence)")
+    ap.add_argument("--policy", default="at_end", choices=("every_k", "at_end"),
+                    help="FP32 forms: refresh mid-call (every_k) or after the call (at_end)")
     ap.add_argument("--k-extra", default="16,64,128",
                     help="further FP32 refresh intervals reported beside the headline")
     ap.add_argument("--seed", type=int, default=20261202)
@@ -458,8 +461,9 @@ def main():
         "config": {
             "workload": f"pi-flux Kagome L={L} ({ham.ns} sites, {n}/{n}), {args.walkers} walkers, "
                         f"step = {args.sweeps_per_step or n} sweeps + 1 E_loc; FP32 W, delay {args.delay}, "
-                        f"FP64 refresh every {args.k} accepts",
+                        f"FP64 refresh every {args.k} accepts ({args.policy})",
             "L": L, "walkers": args.walkers, "delay": args.delay, "refresh_k": args.k,
+            "refresh_policy": args.policy,
         },
         "roofline": dominant,
         "roofline_other": rs if dominant is rr else rr,

@@ -53,6 +53,12 @@ extern "C" {
  * EVERY_K: after every refresh_k accepted moves of a walker. */
 #define KVMC_REFRESH_REFERENCE 0
 #define KVMC_REFRESH_EVERY_K 1
+/* AT_END (no reference counterpart): a walker that has made refresh_k
+ * accepted moves since its last refresh finishes the sweeps of the current
+ * kvmc_sweep call and is refreshed at its end (at most one call's accepts
+ * late).  Saves the second sweep launch that EVERY_K needs for walkers
+ * paused mid-call. */
+#define KVMC_REFRESH_AT_END 2
 
 /* Per-sweep outcome codes written by kvmc_sweep (optional record). */
 #define KVMC_OUT_PRE_REJECT 0 /* r > Z_mu / Z_max (src/MonteCarlo.jl:547-550) */
@@ -169,7 +175,7 @@ int kvmc_get_sweep_count(kvmc_handle* h, int64_t* sweeps);
  * clusters. */
 int kvmc_enable_graph(kvmc_handle* h, int32_t enable);
 
-/* EVERY_K policy only (no effect otherwise): sets each walker's count of
+/* EVERY_K and AT_END policies (no effect otherwise): sets each walker's count of
  * accepted moves since its last refresh to a seeded uniform draw in
  * [0, refresh_k), so that refreshes are spread over time as in a long run
  * instead of arriving together after a common start.  Decisions of the

@@ -318,9 +318,10 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         return fail(KVMC_ERR_INVALID, "unknown precision");
     if (d->kernel != KVMC_KERNEL_REFERENCE && d->kernel != KVMC_KERNEL_UNBIASED)
         return fail(KVMC_ERR_INVALID, "unknown transition kernel");
-    if (d->refresh_policy != KVMC_REFRESH_REFERENCE && d->refresh_policy != KVMC_REFRESH_EVERY_K)
+    if (d->refresh_policy != KVMC_REFRESH_REFERENCE && d->refresh_policy != KVMC_REFRESH_EVERY_K &&
+        d->refresh_policy != KVMC_REFRESH_AT_END)
         return fail(KVMC_ERR_INVALID, "unknown refresh policy");
-    if (d->refresh_policy == KVMC_REFRESH_EVERY_K && d->refresh_k <= 0)
+    if (d->refresh_policy != KVMC_REFRESH_REFERENCE && d->refresh_k <= 0)
         return fail(KVMC_ERR_INVALID, "refresh_k must be positive");
     const int delay = d->delay <= 1 ? 1 : d->delay;
     if (delay > kMaxDelay)
@@ -551,7 +552,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
     int64_t launches = 0;
     hipStream_t s = h->stream;
     const bool use_graph = h->graph && !tape_u && !outcomes && !h->profiling;
-    bool resume = false;   // walkers are paused and await a refresh
+    bool finished = false;   // the replay left no walker paused
     if (use_graph) {
         // One replay: stop counts, counter and flag resets (fill kernels, no
         // memset nodes), then kGraphRounds rounds of sweep + refresh of the
@@ -604,10 +605,13 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
             return fail(KVMC_ERR_SINGULAR, "tildeU is singular for walker " + std::to_string(failed) +
                                                " (lu factorization failed)");
         }
-        if (count == 0) goto done;
-        rc = refresh_list(h, h->rlist, count, 1);
-        if (rc) { cleanup(); return rc; }
-        resume = true;
+        if (count == 0) {
+            finished = true;
+        } else {
+            rc = refresh_list(h, h->rlist, count, 1);
+            if (rc) { cleanup(); return rc; }
+            finished = h->policy == KVMC_REFRESH_AT_END;
+        }
     } else {
         hipLaunchKernelGGL(k_set_stop, dim3((h->nw + 255) / 256), dim3(256), 0, s,
                            h->sweeps, h->stop, h->tbase, (long long)nsweeps, h->nw);
@@ -615,8 +619,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
             cleanup(); return fail(KVMC_ERR_HIP, "counter reset failed");
         }
     }
-    (void)resume;
-    for (;;) {
+    while (!finished) {
         int count = 0;
         if (hipMemsetAsync(h->rcount, 0, sizeof(int), s) != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, "hipMemsetAsync failed"); }
         prof_begin(h);
@@ -634,8 +637,8 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         if (count == 0) break;
         rc = refresh_list(h, h->rlist, count, 1);
         if (rc) { cleanup(); return rc; }
+        if (h->policy == KVMC_REFRESH_AT_END) break;   // listed walkers had finished
     }
-done:
     if (outcomes &&
         hipMemcpyAsync(outcomes, d_out, tape_n, hipMemcpyDeviceToHost, s) != hipSuccess) {
         cleanup(); return fail(KVMC_ERR_HIP, "outcome copy failed");
@@ -794,7 +797,7 @@ int kvmc_enable_graph(kvmc_handle* h, int32_t enable) {
 int kvmc_stagger_refresh(kvmc_handle* h, uint64_t seed) {
     g_err.clear();
     if (!h) return fail(KVMC_ERR_INVALID, "null handle");
-    if (h->policy != KVMC_REFRESH_EVERY_K) return KVMC_OK;
+    if (h->policy == KVMC_REFRESH_REFERENCE) return KVMC_OK;
     std::vector<int> since(h->nw);
     for (int w = 0; w < h->nw; ++w) {
         uint32_t x[4];

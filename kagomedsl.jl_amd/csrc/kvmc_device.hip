@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
             __syncthreads();
             zmu += tdz;
             refresh_due = a.policy == POLICY_REFERENCE ? (t_sw % a.nocc) == 0
-                                                       : acc_since >= a.refresh_k;
+                        : a.policy == POLICY_EVERY_K ? acc_since >= a.refresh_k
+                                                     : false;   // AT_END: listed at exit
         } else {
             refresh_due = true;   // rejected move on a reference-cadence refresh sweep
         }
@@ -390,7 +391,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         a.npend[w] = c;
         a.acc_since[w] = acc_since;
         a.zmu[w] = zmu;
-        if (pause) {
+        // AT_END: a walker that reached refresh_k accepts during this call
+        // finishes its sweeps and is refreshed after the launch
+        if (pause || (a.policy == POLICY_AT_END && acc_since >= a.refresh_k)) {
             int slot = atomicAdd(a.refresh_count, 1);
             a.refresh_list[slot] = w;
             atomicAdd(&a.counters[CNT_PAUSES], 1ull);

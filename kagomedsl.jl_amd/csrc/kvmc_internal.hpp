@@ -10,7 +10,7 @@ namespace kvmc {
 // Per-sweep outcome codes (mirror of KVMC_OUT_* in kagome_vmc.h).
 enum : int { OUT_PRE_REJECT = 0, OUT_NO_MOVE = 1, OUT_REJECT = 2, OUT_ACCEPT = 3 };
 enum : int { KERNEL_REFERENCE = 0, KERNEL_UNBIASED = 1 };
-enum : int { POLICY_REFERENCE = 0, POLICY_EVERY_K = 1 };
+enum : int { POLICY_REFERENCE = 0, POLICY_EVERY_K = 1, POLICY_AT_END = 2 };
 
 // Device-wide counters (u64 slots).
 enum : int { CNT_MOVES = 0, CNT_ACCEPTED = 1, CNT_FLUSHES = 2, CNT_SWEEPS = 3, CNT_NAP = 4,

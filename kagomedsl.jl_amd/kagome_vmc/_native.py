@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libkagome_vmc.so")
 KVMC_OK, KVMC_ERR_INVALID, KVMC_ERR_SINGULAR, KVMC_ERR_HIP, KVMC_ERR_NOMEM, KVMC_ERR_NOT_READY = range(6)
 PREC_FP64, PREC_FP32 = 0, 1
 KERNEL_REFERENCE, KERNEL_UNBIASED = 0, 1
-REFRESH_REFERENCE, REFRESH_EVERY_K = 0, 1
+REFRESH_REFERENCE, REFRESH_EVERY_K, REFRESH_AT_END = 0, 1, 2
 OUT_PRE_REJECT, OUT_NO_MOVE, OUT_REJECT, OUT_ACCEPT = 0, 1, 2, 3
 
 EXPORTS = [

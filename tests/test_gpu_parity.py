@@ -331,3 +331,28 @@ def test_graph_replay_matches_launch_loop(oracle_mod, L, prec, policy, k):
             assert c0[key] == c1[key], key
         assert np.array_equal(e0, e1)
         assert np.array_equal(k0[0], k1[0]) and np.array_equal(k0[1], k1[1])
+
+
+def test_refresh_at_end_policy(oracle_mod):
+    """AT_END refreshes a walker after the call in which it reaches k
+    accepts: decisions equal those of EVERY_K (FP64 W), one sweep launch per
+    call, and no walker is refreshed more than once per call."""
+    ham = system(4)
+    nw = 64
+    ku, kd = well_conditioned_configs(oracle_mod, 41, nw, ham)
+    out = {}
+    for policy in (1, 2):
+        with sampler(ham, nw, refresh_policy=policy, refresh_k=8, seed=17) as ds:
+            ds.set_configs(ku, kd)
+            rec = []
+            for _ in range(3):
+                cnt, _ = ds.sweep(48)
+                rec.append(cnt)
+            out[policy] = (rec, ds.get_configs(), ds.measure()[0])
+    (r1, c1, e1), (r2, c2, e2) = out[1], out[2]
+    assert np.array_equal(c1[0], c2[0]) and np.array_equal(c1[1], c2[1])
+    assert np.allclose(e1, e2, rtol=1e-10, atol=1e-10)
+    for a, b in zip(r1, r2):
+        assert a["accepted"] == b["accepted"]
+        assert b["launches"] == 1 and 0 < b["refreshes"] <= nw
+        assert a["refreshes"] >= b["refreshes"]
