@@ -41,9 +41,9 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) per
 # walker refreshed by k_refresh_blocked, from rocprofv3 --pmc passes over this
 # script's default FP32 run (summaries in profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 16_782_627}
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 16_711_265}
 # the same for k_sweep, per walker-sweep (FP32, delay 8, L=12)
-SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 34_016}
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 33_902}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),
