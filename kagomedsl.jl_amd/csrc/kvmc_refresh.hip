@@ -1,22 +1,32 @@
 // W refresh (reevaluateW!, src/MonteCarlo.jl:55-66) as a blocked FP64
-// factorisation on gfx950: one 256-thread workgroup per (walker, spin).
+// factorisation on gfx950.
 //
 // Work matrix (row-major, zero padded): M = [ tildeU ; U_empty ], ns x N.
 // Physical rows 0..N-1 are U at the occupied sites ordered by label (tildeU,
 // as in tilde_U, src/MonteCarlo.jl:92-115); rows N..ns-1 are U at the empty
 // sites.  Row interchanges are implicit: phys[] maps the logical (pivoted)
 // row order onto physical rows, so no row of M ever moves.
-//   1. Right-looking LU with partial pivoting restricted to rows < N, panels
-//      of RB = 32 columns.  The panel lives in registers (one row per
-//      thread); per pivot column one barrier publishes each wavefront's best
-//      candidate row.  U12 = L11^-1 A12 and the trailing update both run on
-//      FP64 MFMA (v_mfma_f64_16x16x4f64), L11^-1 being the explicit inverse
-//      of the unit-lower 32x32 block.  Logical rows >= N end as
-//      L2 = U_empty R^-1.
-//   2. X L1 = L2 (unit lower, right side), panels right to left: X_J = B_J
-//      L1_JJ^-1 and the update of the remaining columns, both on MFMA.
-//   3. W_empty = X P; occupied rows of W are unit rows.  Written coalesced per
-//      label column, rounded to the storage precision.
+//
+// Two paths share the LU code (k_refresh_blocked, one 256-thread workgroup
+// per (walker, spin)):
+//
+// N <= 256: two kernels.
+//   k_refresh_blocked<FULL = false> factors tildeU alone: P tildeU = L1 U1,
+//   right-looking, panels of RB = 32 columns; the panel lives in registers
+//   (one row per thread) and one barrier per pivot column publishes each
+//   wavefront's best candidate row.  U12 = L11^-1 A12 and the trailing
+//   update run on FP64 MFMA (v_mfma_f64_16x16x4f64).  It saves L1, U1, the
+//   32x32 inverses of the diagonal blocks of L1 and U1 and the final row map.
+//   k_refresh_solve then forms the empty-site rows of W: every wavefront
+//   holds 16 empty sites, i.e. their rows b of U transposed, as 16x16 MFMA
+//   accumulator tiles in registers, and solves y U1 = b and x L1 = y in place
+//   (block rows of the factors staged through LDS, shared by the four
+//   wavefronts).  W[s][phys[c]] = x[c]; occupied rows of W are unit rows.
+//
+// N > 256 (register tiles would not fit): one kernel, k_refresh_blocked<FULL
+// = true>, factors the tall matrix [tildeU ; U_empty] (rows >= N end as
+// L2 = U_empty U1^-1), solves X L1 = L2 panel by panel on MFMA and writes
+// W_empty = X P through LDS.
 #include "kvmc_internal.hpp"
 
 #include <math.h>
@@ -39,10 +49,22 @@ constexpr int LB = RB + 1;  // LDS row stride of 32-wide blocks
 
 __host__ __device__ inline int rup_i(int x, int m) { return (x + m - 1) / m * m; }
 __host__ __device__ inline int refresh_nrows(int ns) { return rup_i(ns, 16) + 16; }
-__host__ __device__ inline int refresh_ldm(int nmax) { return rup_i(nmax, 32); }
-// Work doubles per job: the padded tall matrix plus the saved L11 inverses.
+// Register tiles per wavefront of k_refresh_solve for N <= 256: 4, 8, 14 or
+// 16 tiles of 16 labels (0: N > 256).
+__host__ __device__ inline int solve_nt(int nmax) {
+    const int nt = (nmax + 15) / 16;
+    return nt <= 4 ? 4 : nt <= 8 ? 8 : nt <= 14 ? 14 : nt <= 16 ? 16 : 0;
+}
+// Row stride of M: whole 16 * NT rows for the solve kernel's row loads.
+__host__ __device__ inline int refresh_ldm(int nmax) { return solve_nt(nmax) ? 16 * solve_nt(nmax) : rup_i(nmax, 32); }
+__host__ __device__ inline int refresh_npanel(int nmax) { return (nmax + RB - 1) / RB; }
+// Work doubles per job: the padded tall matrix, the saved 32x32 inverses of
+// the diagonal blocks of L1 and of U1, then ints: [0] factorisation ok flag,
+// [1..nmax] logical row -> label.  Rounded to 256 bytes.
 __host__ __device__ inline size_t refresh_ws_job(int ns, int nmax) {
-    return (size_t)refresh_nrows(ns) * refresh_ldm(nmax) + (size_t)((nmax + RB - 1) / RB) * RB * RB;
+    const size_t d = (size_t)refresh_nrows(ns) * refresh_ldm(nmax) + 2 * (size_t)refresh_npanel(nmax) * RB * RB +
+                     (size_t)(nmax + 2) / 2 + 1;
+    return (d + 31) / 32 * 32;
 }
 
 __device__ inline void atomic_max_nonneg2(double* addr, double v) {
@@ -156,7 +178,7 @@ __device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
     return (b & ~0xFFFull) | (unsigned long long)(4095 - row);
 }
 
-template <typename T, int RPT>
+template <typename T, int RPT, bool FULL>
 __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_refresh_blocked(RefreshArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int job = (int)blockIdx.x;
@@ -173,7 +195,12 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     const int* __restrict__ kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
     double* __restrict__ M = a.ws + (size_t)job * refresh_ws_job(ns, nmax);
     double* __restrict__ Lsave = M + (size_t)nrows * ldm;   // [npanel][RB][RB] inverses of L11
+    double* __restrict__ Usave = Lsave + (size_t)refresh_npanel(nmax) * RB * RB;   // inverses of U11
+    int* __restrict__ jint = reinterpret_cast<int*>(Usave + (size_t)refresh_npanel(nmax) * RB * RB);
     const int t = (int)threadIdx.x, wave = t / 64, lane = t % 64;
+    if (!FULL && t == 0) jint[0] = 0;                       // set to 1 once the factors are complete
+    // logical rows carrying data: all of M, or only tildeU
+    const int nrl = FULL ? nr16 : rup_i(N, 16);
 
     // ---- LDS carve-up ----------------------------------------------------
     double* cand = reinterpret_cast<double*>(lds);          // [2][NWAVE][RB]
@@ -233,11 +260,16 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
         if (t == 0) { a.status[w] = 2; atomicMin(a.failed, w); }
         return;
     }
+    if (!FULL) {
+        // columns nc16..ldm-1 of M are read (as zeros) by the solve kernel
+        const int nz = ldm - nc16;
+        for (int idx = t; idx < nrl * nz; idx += RT) M[(size_t)(idx / nz) * ldm + nc16 + idx % nz] = 0.0;
+    }
     // M is never filled from U up front: the first panel (j0 = 0) reads its
     // operands straight from U (L2-resident, shared by every job) and every
     // later read of M finds values written by an earlier step.
     auto u_at = [&](int prow, int col) -> double {
-        return (prow < ns && col < N) ? UT[(size_t)rowsite[prow] * N + col] : 0.0;
+        return (prow < (FULL ? ns : N) && col < N) ? UT[(size_t)rowsite[prow] * N + col] : 0.0;
     };
 
     // ---- 1. blocked LU ---------------------------------------------------
@@ -333,11 +365,22 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
         for (int i = jb + t; i < RB; i += RT) {
             for (int c = 0; c < RB; ++c) Dblk[i * LB + c] = (c == i) ? 1.0 : 0.0;
         }
+        if (!FULL) {
+            // padding rows N..nrl-1 (phys[r] = r) enter the trailing update
+            // as zero rows; nothing reads their results
+            for (int r = N + t; r < nrl; r += RT) {
+                for (int c = 0; c < RB; ++c) M[(size_t)r * ldm + j0 + c] = 0.0;
+            }
+        }
         __syncthreads();
-        // empty-site rows: their panel slice solves x U11 = a  ->  x = a U11^-1
+        // U11^-1: kept for the solve kernel, or applied here to the empty-site
+        // rows, whose panel slice solves x U11 = a  ->  x = a U11^-1
         upper_inverse(Dblk, Linv);
         __syncthreads();
-        {
+        if (!FULL) {
+            for (int idx = t; idx < RB * RB; idx += RT)
+                Usave[(size_t)(j0 / RB) * RB * RB + idx] = Linv[(idx / RB) * LB + idx % RB];
+        } else {
             const int ntr = m16 / 16;
             for (int ti = wave; ti < ntr; ti += NWAVE) {
                 const int ra = N + 16 * ti + (lane & 15);
@@ -393,7 +436,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
         __syncthreads();
         // trailing update: M[r][c] -= L21[r][:] U12[:][c]
         const int r0 = c0;
-        const int ntr = (nr16 - r0) / 16;
+        const int ntr = (nrl - r0) / 16;
         for (int ti = wave; ti < ntr; ti += NWAVE) {
             const int pa = phys[r0 + 16 * ti + (lane & 15)];
             double af[RB / 4];
@@ -459,6 +502,12 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     if (t == 0) {
         a.logdet[(size_t)w * 2 + spin] = logdet;
         if (a.write_w && spin == 0) { a.npend[w] = 0; a.acc_since[w] = 0; }
+    }
+    if constexpr (!FULL) {
+        // logical row c < N holds label phys[c]; W is formed by k_refresh_solve
+        for (int c = t; c < N; c += RT) jint[1 + c] = phys[c];
+        if (t == 0) jint[0] = 1;
+        return;
     }
     // logical row c < N holds label phys[c]
     for (int c = t; c < N; c += RT) pinv[phys[c]] = c;
@@ -581,6 +630,251 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     }
 }
 
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Stages row block J of the factors (logical rows 16J..16J+15 of M, NT * 16
+// columns; row stride LDBK in LDS) into buf and the 16x16 diagonal block of
+// the saved 32x32 inverse `inv` into dbuf with direct global-to-LDS loads
+// (global_load_lds_dwordx4: lane l's 16 bytes land at dst + 16 l).  buf is
+// 2 (NT + 1) chunks of 1 KiB and dbuf 2, dealt to the wavefronts in turn;
+// the LDS row padding receives a copy of the row start, never read.  The
+// loads are tracked by vmcnt, so the barrier that ends a block's step also
+// completes the next block's staging.
+template <int NT>
+__device__ __forceinline__ void solve_stage(const double* M, int ldm, const int* lab, const double* inv, int J,
+                                            double* buf, double* dbuf) {
+    constexpr int LDBK = NT * 16 + 16;
+    constexpr int NCH = 2 * (NT + 1);
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    for (int c = wave; c < NCH + 2; c += NWAVE) {
+        if (c < NCH) {
+            const int o = c * 128 + lane * 2;
+            const int r = o / LDBK, col = o % LDBK;
+            const double* row = M + (size_t)lab[16 * J + r] * ldm;
+            __builtin_amdgcn_global_load_lds(col < NT * 16 ? row + col : row, (lds_void*)(buf + c * 128), 16, 0, 0);
+        } else {
+            const int o = (c - NCH) * 128 + lane * 2;
+            const int h = (J & 1) * 16;
+            const double* src = inv + (size_t)(J >> 1) * RB * RB + (h + o / 16) * RB + h + o % 16;
+            __builtin_amdgcn_global_load_lds(src, (lds_void*)(dbuf + (c - NCH) * 128), 16, 0, 0);
+        }
+    }
+}
+
+// The empty-site rows of W from the factors of k_refresh_blocked<FULL =
+// false>: W[s][:] = U[s][:] tildeU^-1 = ((U[s][:] U1^-1) L1^-1) P.
+//
+// Each wavefront takes 16 empty sites and keeps the transposed rows as
+// 16x16 tiles T_J (labels 16J.. of the 16 sites) in MFMA accumulator layout:
+// lane l, register g holds T_J[l / 16 + 4 g][l % 16].  That layout is also
+// the B-operand layout of the four k-slices of v_mfma_f64_16x16x4f64
+// (slice s = register s), so a solved tile feeds the next products without
+// leaving registers:
+//   y U1 = b  <=>  U1^T y^T = b^T, J ascending:
+//     T_J = (U1_JJ^-1)^T T_J,  then T_K -= U1[J,K]^T T_J for K > J;
+//   x L1 = y  <=>  L1^T x^T = y^T, J descending:
+//     T_J = (L1_JJ^-1)^T T_J,  then T_K -= L1[J,K]^T T_J for K < J.
+// Row block J of the factors (16 x N) and the 16x16 diagonal inverse are
+// staged in LDS, double buffered: the next block streams in while the
+// current one is used, one barrier per block.  Step q of the 2 nt steps
+// reads buffer q & 1; q = J in the first solve and 2 nt - 1 - J in the
+// second, so the buffer is (J + 1) & 1 there, known at compile time.
+//
+// Grid: 8 * groups * ceil(njobs / 8) workgroups of 256 threads; a
+// workgroup takes 64 consecutive empty sites (in site order) of one job,
+// and the groups of one job share blockIdx % 8, i.e. one XCD and its L2.
+// It also writes the unit rows of the occupied sites between its first
+// empty site and the next group's, so every entry of W is written once.
+template <typename T, int NT>
+__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2)))
+k_refresh_solve(RefreshArgs a, int groups) {
+    constexpr int LDBK = NT * 16 + 16;   // LDS row stride: row groups 1 and 2 land 32 banks apart
+    __shared__ __align__(16) double sb0[16 * LDBK], sb1[16 * LDBK];
+    __shared__ __align__(16) double sd0[256], sd1[256];
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int b = (int)blockIdx.x;
+    const int job = ((b >> 3) / groups) * 8 + (b & 7);
+    const int grp = (b >> 3) % groups;
+    if (job >= a.njobs) return;
+    if (a.count_dev && job >= 2 * (*a.count_dev - a.walker_off)) return;
+    const int w = a.jobs[job >> 1], spin = job & 1;
+    const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
+    const int nE = ns - N;
+    if (grp > 0 && 64 * grp >= nE) return;
+    const int nt = (N + 15) / 16;
+    const double* __restrict__ M = a.ws + (size_t)job * refresh_ws_job(ns, nmax);
+    const int ldm = refresh_ldm(nmax);
+    const double* __restrict__ Lsave = M + (size_t)refresh_nrows(ns) * ldm;
+    const double* __restrict__ Usave = Lsave + (size_t)refresh_npanel(nmax) * RB * RB;
+    const int* __restrict__ jint = reinterpret_cast<const int*>(Usave + (size_t)refresh_npanel(nmax) * RB * RB);
+    if (jint[0] != 1) return;                     // factorisation rejected (status already set)
+    const int* __restrict__ kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
+    const double* __restrict__ Ucm = spin ? a.Udn : a.Uup;     // ns x N column-major
+    const int t = (int)threadIdx.x, wave = t / 64, lane = t % 64;
+
+    int* lab = reinterpret_cast<int*>(lds);       // [NT*16] logical row -> physical row of M
+    int* wsum = lab + NT * 16;                    // [NWAVE]
+    int* esite = wsum + NWAVE;                    // [ns] empty sites in order
+
+    // rows N..16 nt - 1 are the zeroed padding rows of M
+    for (int c = t; c < NT * 16; c += RT) lab[c] = c < N ? jint[1 + c] : c;
+    int base = 0;
+    for (int s0 = 0; s0 < ns; s0 += RT) {
+        const int s = s0 + t;
+        const bool empty = s < ns && kap[s] == 0;
+        const unsigned long long m = __ballot(empty);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base, tot = 0;
+        for (int q = 0; q < NWAVE; ++q) {
+            off += q < wave ? wsum[q] : 0;
+            tot += wsum[q];
+        }
+        if (empty && off + before < nE) esite[off + before] = s;   // count checked by the LU kernel
+        base += tot;
+        __syncthreads();
+    }
+
+    const int e = 64 * grp + 16 * wave + (lane & 15);
+    const int site = e < nE ? esite[e] : -1;
+    const bool active = 64 * grp + 16 * wave < nE;   // wavefront holds at least one site
+    const int kr = lane >> 4, kc = lane & 15;
+
+    // The tiles hold the accumulators negated: T_K = -acc_K.  The diagonal
+    // step then reads the inverse negated (4 values per step) and the
+    // updates T_K += F[J,K]^T y_J need no sign change per block.
+    f64x4 tile[NT];
+#pragma unroll
+    for (int J = 0; J < NT; ++J) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int c = 16 * J + kr + 4 * g;
+            tile[J][g] = (site >= 0 && c < N) ? -Ucm[(size_t)c * ns + site] : 0.0;
+        }
+    }
+    // A operands of block column K of the staged row block
+    auto lds_a = [&](const double* bb, int K, double (&v)[4]) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[s] = bb[(4 * s + kr) * LDBK + 16 * K + kc];
+    };
+    // T_J = y = (-inv)^T T_J from the staged diagonal inverse
+    auto diag = [&](const double* db, int J) -> f64x4 {
+        double d[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) d[s] = -db[(4 * s + kr) * 16 + kc];
+        f64x4 y = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) y = mfma16(d[s], tile[J][s], y);
+        return y;
+    };
+    // T_K += F[J,K]^T y for K in [k0, k1), two blocks at a time so that
+    // consecutive products are independent; operands one pair ahead
+    auto update = [&](const double* bb, int k0, int k1, const f64x4& y) {
+        double oa[2][4], ob[2][4];
+        if (k0 < k1) lds_a(bb, k0, oa[0]);
+        if (k0 + 1 < k1) lds_a(bb, k0 + 1, ob[0]);
+#pragma unroll
+        for (int P = 0; k0 + 2 * P < k1; ++P) {
+            const int K = k0 + 2 * P, pp = P & 1;
+            if (K + 2 < k1) lds_a(bb, K + 2, oa[pp ^ 1]);
+            if (K + 3 < k1) lds_a(bb, K + 3, ob[pp ^ 1]);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                tile[K] = mfma16(oa[pp][s], y[s], tile[K]);
+                if (K + 1 < k1) tile[K + 1] = mfma16(ob[pp][s], y[s], tile[K + 1]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+
+    solve_stage<NT>(M, ldm, lab, Usave, 0, sb0, sd0);
+    __syncthreads();
+    // y U1 = b, block rows ascending (step J reads buffer J & 1).  Blocks
+    // K >= nt are zero padding, so the updates run to NT without guards.
+#pragma unroll
+    for (int J = 0; J < NT; ++J) {
+        if (J < nt) {
+            double* nb = (J & 1) ? sb0 : sb1;
+            double* nd = (J & 1) ? sd0 : sd1;
+            if (J + 1 < nt) solve_stage<NT>(M, ldm, lab, Usave, J + 1, nb, nd);
+            else solve_stage<NT>(M, ldm, lab, Lsave, nt - 1, nb, nd);
+            const double* bb = (J & 1) ? sb1 : sb0;
+            const double* db = (J & 1) ? sd1 : sd0;
+            if (active) {
+                const f64x4 y = diag(db, J);
+                tile[J] = y;
+                update(bb, J + 1, NT, y);
+            }
+            __syncthreads();
+        }
+    }
+    // the second solve starts from T_K = -y_K
+#pragma unroll
+    for (int J = 0; J < NT; ++J) tile[J] = -tile[J];
+    // x L1 = y, block rows descending (step reads buffer (J + 1) & 1)
+#pragma unroll
+    for (int J = NT - 1; J >= 0; --J) {
+        if (J < nt) {
+            if (J > 0) solve_stage<NT>(M, ldm, lab, Lsave, J - 1, (J & 1) ? sb1 : sb0, (J & 1) ? sd1 : sd0);
+            const double* bb = (J & 1) ? sb0 : sb1;
+            const double* db = (J & 1) ? sd0 : sd1;
+            if (active) {
+                const f64x4 x = diag(db, J);
+                tile[J] = x;
+                update(bb, 0, J, x);
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- W: the solved tiles, then the unit rows of the occupied sites ----
+    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
+    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+    const int nsp = rup_i(ns, 4);
+    const int cpend = a.write_w ? 0 : a.npend[w];
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
+    double dmax = 0.0, vmax = 0.0;
+    auto put = [&](int l, int s, double v) {
+        T* __restrict__ dst = Wo + (size_t)l * ns + s;
+        if (a.write_w) {
+            *dst = (T)v;
+        } else {
+            double cur = (double)*dst;
+            for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + s] * (double)gB[c * nmax + l];
+            dmax = fmax(dmax, fabs(v - cur));
+            vmax = fmax(vmax, fabs(v));
+        }
+    };
+    if (site >= 0) {
+#pragma unroll
+        for (int J = 0; J < NT; ++J) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int c = 16 * J + kr + 4 * g;
+                if (c < N) put(lab[c], site, tile[J][g]);
+            }
+        }
+    }
+    const int s_lo = grp == 0 ? 0 : esite[64 * grp];
+    const int s_hi = 64 * (grp + 1) < nE ? esite[64 * (grp + 1)] : ns;
+    const int span = s_hi - s_lo;
+    for (int idx = t; idx < N * span; idx += RT) {
+        const int l = idx / span, s = s_lo + idx % span;
+        const int ks = kap[s];
+        if (ks != 0) put(l, s, (ks - 1 == l) ? 1.0 : 0.0);
+    }
+    if (!a.write_w) {
+        atomic_max_nonneg2(&a.drift[0], dmax);
+        atomic_max_nonneg2(&a.drift[1], vmax);
+    }
+}
+
+// LDS of k_refresh_solve: static row-block buffers plus the dynamic part.
+static size_t solve_static_lds(int nt) { return (size_t)(2 * 16 * (nt * 16 + 16) + 2 * 256) * sizeof(double); }
+static size_t solve_lds_bytes(int ns, int nt) { return (size_t)(nt * 16 + NWAVE + ns) * sizeof(int); }
+
 size_t refresh_ws_doubles_per_job(int ns, int nmax) { return refresh_ws_job(ns, nmax); }
 
 static size_t refresh_lds_bytes(int ns, int nup, int ndn) {
@@ -598,17 +892,69 @@ static int refresh_rows_per_thread(int nmax) {
     return (nmax + RT - 1) / RT;
 }
 
-template <int RPT>
-static hipError_t launch_refresh_r(int prec, const RefreshArgs& a, size_t lds, hipStream_t s) {
-    const void* fn = prec == 0 ? reinterpret_cast<const void*>(&k_refresh_blocked<double, RPT>)
-                               : reinterpret_cast<const void*>(&k_refresh_blocked<float, RPT>);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+static hipError_t set_lds_limit(const void* fn, size_t lds) {
+    if (lds <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <typename T, int RPT, bool FULL>
+static hipError_t launch_lu(const RefreshArgs& a, size_t lds, hipStream_t s) {
+    hipError_t e = set_lds_limit(reinterpret_cast<const void*>(&k_refresh_blocked<T, RPT, FULL>), lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_refresh_blocked<T, RPT, FULL>), dim3(a.njobs), dim3(RT), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int NT>
+static hipError_t launch_solve(const RefreshArgs& a, hipStream_t s) {
+    const int nmin = a.nup < a.ndn ? a.nup : a.ndn;
+    const int groups = (a.ns - nmin + 63) / 64 > 1 ? (a.ns - nmin + 63) / 64 : 1;
+    const size_t lds = solve_lds_bytes(a.ns, NT);
+    if (solve_static_lds(NT) + lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_refresh_solve<T, NT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    if (prec == 0) hipLaunchKernelGGL((k_refresh_blocked<double, RPT>), dim3(a.njobs), dim3(RT), lds, s, a);
-    else hipLaunchKernelGGL((k_refresh_blocked<float, RPT>), dim3(a.njobs), dim3(RT), lds, s, a);
+    const int grid = 8 * groups * ((a.njobs + 7) / 8);
+    hipLaunchKernelGGL((k_refresh_solve<T, NT>), dim3(grid), dim3(RT), lds, s, a, groups);
     return hipGetLastError();
+}
+
+// Register tiles per wavefront of the solve kernel, or 0 for the
+// single-kernel path (N > 256, or LDS beyond 160 KiB).
+static int solve_tiles(int ns, int nmax) {
+    const int NT = solve_nt(nmax);
+    if (NT == 0 || solve_static_lds(NT) + solve_lds_bytes(ns, NT) > 160 * 1024) return 0;
+    return NT;
+}
+
+template <typename T>
+static hipError_t launch_refresh_t(const RefreshArgs& a, size_t lds, hipStream_t s) {
+    switch (solve_tiles(a.ns, a.nmax)) {
+        case 4: {
+            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            return e != hipSuccess ? e : launch_solve<T, 4>(a, s);
+        }
+        case 8: {
+            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            return e != hipSuccess ? e : launch_solve<T, 8>(a, s);
+        }
+        case 14: {
+            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            return e != hipSuccess ? e : launch_solve<T, 14>(a, s);
+        }
+        case 16: {
+            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            return e != hipSuccess ? e : launch_solve<T, 16>(a, s);
+        }
+        default:
+            break;
+    }
+    switch (refresh_rows_per_thread(a.nmax)) {
+        case 1: return launch_lu<T, 1, true>(a, lds, s);
+        case 2: return launch_lu<T, 2, true>(a, lds, s);
+        default: return launch_lu<T, 4, true>(a, lds, s);
+    }
 }
 
 bool refresh_supported(int ns, int nup, int ndn) {
@@ -618,11 +964,7 @@ bool refresh_supported(int ns, int nup, int ndn) {
 hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s) {
     const size_t lds = refresh_lds_bytes(a.ns, a.nup, a.ndn);
     if (!refresh_supported(a.ns, a.nup, a.ndn)) return hipErrorInvalidValue;
-    switch (refresh_rows_per_thread(a.nmax)) {
-        case 1: return launch_refresh_r<1>(prec, a, lds, s);
-        case 2: return launch_refresh_r<2>(prec, a, lds, s);
-        default: return launch_refresh_r<4>(prec, a, lds, s);
-    }
+    return prec == 0 ? launch_refresh_t<double>(a, lds, s) : launch_refresh_t<float>(a, lds, s);
 }
 
 }  // namespace kvmc

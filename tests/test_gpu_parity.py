@@ -41,6 +41,53 @@ def test_refresh_matches_oracle(oracle_mod, L, prec):
             assert abs(logdet[w] - (lu + ld)) < 1e-9 * max(1.0, abs(lu + ld))
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_refresh_above_256_electrons(oracle_mod, prec):
+    """N_up = 264 at L=12 (432 sites, N_down = 168): beyond 256 electrons of
+    one spin the refresh runs as a single kernel over [tildeU; U_empty].
+    Walker 0 starts from the pivoted-QR configuration, walker 1 from it with
+    20 seeded up/down exchanges, each kept only while both tildeU stay
+    well conditioned (this filling is not a closed shell, so U, and with it
+    the conditioning of a given configuration, depends on the eigensolver)."""
+    from kagome_vmc import DoubleKagome, Hamiltonian, init_conf_qr
+    lat = DoubleKagome.build(1.0, 12, 12, (True, True), (False, True))
+    ham = Hamiltonian.build(264, 168, lat)
+    ku0, kd0 = init_conf_qr(ham.U_up, ham.U_down)
+    ku1, kd1 = ku0.copy(), kd0.copy()
+    rng = np.random.default_rng(5)
+    swaps = 0
+    while swaps < 20:
+        i = rng.choice(np.flatnonzero(ku1))
+        j = rng.choice(np.flatnonzero(kd1))
+        a, b = ku1.copy(), kd1.copy()
+        a[i], a[j] = 0, a[i]
+        b[j], b[i] = 0, b[j]
+        if max(np.linalg.cond(oracle_mod.tilde_u(ham.U_up, a)),
+               np.linalg.cond(oracle_mod.tilde_u(ham.U_down, b))) < 1e5:
+            ku1, kd1 = a, b
+            swaps += 1
+    ku, kd = np.stack([ku0, ku1]), np.stack([kd0, kd1])
+    tol = W_RTOL if prec == 0 else 1e-6
+    with sampler(ham, 2, precision=prec) as ds:
+        ds.set_configs(ku, kd)
+        logdet = ds.logdet()
+        for w in range(2):
+            Wu, lu = oracle_mod.refresh_w(ham.U_up, ku[w], with_logdet=True)
+            Wd, ld = oracle_mod.refresh_w(ham.U_down, kd[w], with_logdet=True)
+            assert _max_rel(ds.get_w(w, 0), Wu.real) < tol
+            assert _max_rel(ds.get_w(w, 1), Wd.real) < tol
+            assert abs(logdet[w] - (lu + ld)) < 1e-9 * max(1.0, abs(lu + ld))
+        if prec == 0:
+            ds.sweep(40)
+            e, _ = ds.measure()
+            ku2, kd2 = ds.get_configs()
+            for w in range(2):
+                Wu = oracle_mod.refresh_w(ham.U_up, ku2[w])
+                Wd = oracle_mod.refresh_w(ham.U_down, kd2[w])
+                eo = oracle_mod.getol(ham.nn, ku2[w], kd2[w], Wu, Wd)
+                assert abs(e[w] - eo) < 1e-8 * max(1.0, abs(eo))
+
+
 @pytest.mark.parametrize("n_up", [25, 30])
 def test_refresh_unequal_filling(oracle_mod, n_up):
     """N_up != N_down (the reference's imbalance scan, scripts/FP.jl:112-116)."""
