@@ -72,13 +72,14 @@ __device__ inline void atomic_max_nonneg2(double* addr, double v) {
               (unsigned long long)__double_as_longlong(v));
 }
 
-// Inverse of the unit-lower jb x jb block L (row stride LB, entries above the
-// diagonal ignored) into Linv (row stride LB), identity beyond jb.  Lanes
-// 0..RB-1 of wavefront 0 each hold one column of the inverse in registers and
-// run its forward substitution fully unrolled (four partial sums per row to
-// shorten the dependency chain); L is read as LDS broadcasts.
-__device__ __attribute__((noinline)) void unit_lower_inverse(const double* L, double* Linv, int jb) {
-    if (threadIdx.x >= 64) return;
+// Inverse of the unit-lower jb x jb block L (row stride LB; the diagonal and
+// the entries above it are not read) into Linv (row stride LB), identity
+// beyond jb.  Lanes 0..RB-1 of wavefront `wv` each hold one column of the
+// inverse in registers and run its forward substitution fully unrolled (four
+// partial sums per row to shorten the dependency chain); L is read as LDS
+// broadcasts.
+__device__ __attribute__((noinline)) void unit_lower_inverse(const double* L, double* Linv, int jb, int wv) {
+    if ((int)threadIdx.x / 64 != wv) return;
     const int j = (int)threadIdx.x & (RB - 1);
     double x[RB];
 #pragma unroll
@@ -97,7 +98,7 @@ __device__ __attribute__((noinline)) void unit_lower_inverse(const double* L, do
         }
         x[i] = v;
     }
-    if (threadIdx.x < RB) {
+    if ((threadIdx.x & 63) < RB) {
 #pragma unroll
         for (int i = 0; i < RB; ++i) Linv[i * LB + j] = x[i];
     }
@@ -114,14 +115,14 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 }
 
 // Inverse of the upper-triangular block D (row stride LB, entries below the
-// diagonal ignored; rows >= jb must be unit rows) into Uinv (row stride LB).
-// Lanes 0..RB-1 of wavefront 0 each back-substitute one column j in
-// registers: x_i = (delta_ij - sum_{k>i} D[i][k] x_k) / D[i][i]; entries
-// below the diagonal come out zero without special cases.  Lane i computes
-// 1/D[i][i] once and the others read it back through readlane.
-__device__ __attribute__((noinline)) void upper_inverse(const double* D, double* Uinv) {
-    if (threadIdx.x >= 64) return;
-    const int lane = (int)threadIdx.x;
+// diagonal ignored; rows >= jb must be unit rows) into Uinv (row stride ld,
+// LDS or global memory).  Lanes 0..RB-1 of wavefront `wv` each back-substitute
+// one column j in registers: x_i = (delta_ij - sum_{k>i} D[i][k] x_k) /
+// D[i][i]; entries below the diagonal come out zero without special cases.
+// Lane i computes 1/D[i][i] once and the others read it back through readlane.
+__device__ __attribute__((noinline)) void upper_inverse(const double* D, double* Uinv, int ld, int wv) {
+    if ((int)threadIdx.x / 64 != wv) return;
+    const int lane = (int)threadIdx.x & 63;
     const int j = lane & (RB - 1);
     const double dinv = 1.0 / D[j * LB + j];
     double x[RB];
@@ -139,7 +140,7 @@ __device__ __attribute__((noinline)) void upper_inverse(const double* D, double*
     }
     if (lane < RB) {
 #pragma unroll
-        for (int i = 0; i < RB; ++i) Uinv[i * LB + j] = x[i];
+        for (int i = 0; i < RB; ++i) Uinv[i * ld + j] = x[i];
     }
 }
 
@@ -304,12 +305,15 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                     }
                 }
                 const unsigned long long wkey = wave_max_u64(key);
-                // the lane owning the wavefront's best row publishes it
+                // the lane owning the wavefront's best row publishes its
+                // entries jj.. (the ones the update reads), 16 bytes a store
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     if (bq == q && key == wkey && wkey != 0ull) {
 #pragma unroll
-                        for (int c = 0; c < RB; ++c) cand[(par * NWAVE + wave) * RB + c] = x[q][c];
+                        for (int c = jj & ~1; c < RB; c += 2)
+                            *reinterpret_cast<double2*>(&cand[(par * NWAVE + wave) * RB + c]) =
+                                make_double2(x[q][c], x[q][c + 1]);
                     }
                 }
                 if (lane == 0) candk[par * NWAVE + wave] = wkey;
@@ -321,10 +325,9 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                     if (k2 > gk) { gk = k2; gw = q; }
                 }
                 const int p = gk ? 4095 - (int)(gk & 0xFFFull) : 0x7fffffff;
-                // lane c holds pivot-row entry c; the update reads them back
-                // through readlane as scalar operands (no per-entry LDS trips)
-                const double pv = (p < N) ? cand[(par * NWAVE + gw) * RB + (lane & (RB - 1))] : 0.0;
-                const double piv = readlane_f64(pv, jj);
+                // the pivot row, read by every lane at the same LDS address
+                const double* prow = cand + (par * NWAVE + gw) * RB;
+                const double piv = (p < N) ? prow[jj] : 0.0;
                 if (!(fabs(piv) > 0.0) || !isfinite(piv)) {
                     singular = true;
                 } else {
@@ -341,7 +344,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                             const double l = x[q][jj] * rp;
                             x[q][jj] = l;
 #pragma unroll
-                            for (int c = jj + 1; c < RB; ++c) x[q][c] -= l * readlane_f64(pv, c);
+                            for (int c = jj + 1; c < RB; ++c) x[q][c] -= l * prow[c];
                         }
                     }
                 }
@@ -373,14 +376,16 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
             }
         }
         __syncthreads();
-        // U11^-1: kept for the solve kernel, or applied here to the empty-site
-        // rows, whose panel slice solves x U11 = a  ->  x = a U11^-1
-        upper_inverse(Dblk, Linv);
-        __syncthreads();
+        // U11^-1: kept for the solve kernel (computed by wavefront 0 while
+        // wavefront 1 forms L11^-1), or applied here to the empty-site rows,
+        // whose panel slice solves x U11 = a  ->  x = a U11^-1
         if (!FULL) {
-            for (int idx = t; idx < RB * RB; idx += RT)
-                Usave[(size_t)(j0 / RB) * RB * RB + idx] = Linv[(idx / RB) * LB + idx % RB];
+            upper_inverse(Dblk, Usave + (size_t)(j0 / RB) * RB * RB, RB, 0);
+            unit_lower_inverse(Dblk, Linv, jb, 1);
+            __syncthreads();
         } else {
+            upper_inverse(Dblk, Linv, LB, 0);
+            __syncthreads();
             const int ntr = m16 / 16;
             for (int ti = wave; ti < ntr; ti += NWAVE) {
                 const int ra = N + 16 * ti + (lane & 15);
@@ -399,15 +404,10 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                     M[row + 16 + (lane & 15)] = y1[g];
                 }
             }
+            __syncthreads();
+            unit_lower_inverse(Dblk, Linv, jb, 0);
+            __syncthreads();
         }
-        __syncthreads();
-        for (int idx = t; idx < RB * RB; idx += RT) {
-            const int i = idx / RB, c = idx % RB;
-            if (c >= i) Dblk[i * LB + c] = 0.0;      // keep the strictly lower L11
-        }
-        __syncthreads();
-        unit_lower_inverse(Dblk, Linv, jb);
-        __syncthreads();
         for (int idx = t; idx < RB * RB; idx += RT)
             Lsave[(size_t)(j0 / RB) * RB * RB + idx] = Linv[(idx / RB) * LB + idx % RB];
         const int c0 = j0 + RB;
