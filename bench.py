@@ -39,11 +39,12 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 # HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) per
-# walker refreshed by k_refresh_blocked, from rocprofv3 --pmc passes over this
-# script's default FP32 run (summaries in profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 16_711_265}
+# walker refreshed by the kernel pair k_refresh_blocked + k_refresh_solve, from
+# rocprofv3 --pmc passes over this script's default FP32 run (summaries in
+# profiles/).
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 6_422_366}
 # the same for k_sweep, per walker-sweep (FP32, delay 8, L=12)
-SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 33_902}
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 33_905}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),
@@ -239,7 +240,8 @@ def roofline_refresh(r, ns, traffic_per_walker):
     return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic_per_walker * wpl if traffic_per_walker else None,
-            "kernel": "k_refresh_blocked (FP64 pivoted LU + MFMA solve, W = U tildeU^-1)",
+            "kernel": "k_refresh_blocked + k_refresh_solve (FP64 pivoted LU of tildeU, then MFMA "
+                      "triangular solves for the empty-site rows of W = U tildeU^-1)",
             "units_per_launch": wpl, "launch_ms": r["refresh_ms_per_launch"]}
 
 
@@ -325,8 +327,9 @@ def run_grad(args):
     gs = GradientSampler(fam, THETA0, n, n, args.walkers, seed=args.seed, precision=nat.PREC_FP64)
     s = gs.sampler
     s.init_random(args.seed)
-    for _ in range(args.warmup):
+    for _ in range(args.warmup):     # warm-up steps do the timed steps' work (first launches included)
         s.sweep(n)
+        s.measure_grad()
     sync()
     t_grad = 0.0
     E, O = [], []
