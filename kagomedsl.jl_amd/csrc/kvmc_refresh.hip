@@ -46,6 +46,7 @@ constexpr int RB = 32;    // panel width
 constexpr int RT = 256;   // threads per workgroup
 constexpr int NWAVE = RT / 64;
 constexpr int LB = RB + 1;  // LDS row stride of 32-wide blocks
+constexpr int LU_WAVES = 3;  // waves per SIMD of the LU-only kernel
 
 __host__ __device__ inline int rup_i(int x, int m) { return (x + m - 1) / m * m; }
 __host__ __device__ inline int refresh_nrows(int ns) { return rup_i(ns, 16) + 16; }
@@ -180,7 +181,7 @@ __device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
 }
 
 template <typename T, int RPT, bool FULL>
-__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_refresh_blocked(RefreshArgs a) {
+__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : LU_WAVES))) k_refresh_blocked(RefreshArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int job = (int)blockIdx.x;
     if (job >= a.njobs) return;
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
     double* Linv = Dblk + RB * LB;                          // [RB][LB] U11^-1, then L11^-1
     const int su = max(nc16 - RB, 16);                      // U12 row stride in LDS
     double* stage = Linv + RB * LB;                         // U12 [RB][su] / scatter block
-    const int stage_len = max(RB * su, 16 * (nc16 | 1));
+    const int stage_len = FULL ? max(RB * su, 16 * (nc16 | 1)) : 0;   // U12 / W staging (single kernel)
     double* pivabs = stage + stage_len;                     // [nmax] |pivot|
     double* redd = pivabs + nmax;                           // [3 * NWAVE]
     int* wsum = reinterpret_cast<int*>(redd + 3 * NWAVE);   // [NWAVE]
@@ -429,8 +430,10 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                 const int r0 = (lane >> 4) + 4 * g, r1 = r0 + 16;
                 M[(size_t)phys[j0 + r0] * ldm + c0 + cc] = u0[g];
                 M[(size_t)phys[j0 + r1] * ldm + c0 + cc] = u1[g];
-                stage[r0 * su + cc] = u0[g];
-                stage[r1 * su + cc] = u1[g];
+                if (FULL) {
+                    stage[r0 * su + cc] = u0[g];
+                    stage[r1 * su + cc] = u1[g];
+                }
             }
         }
         __syncthreads();
@@ -449,6 +452,12 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                 prow[g] = phys[r0 + 16 * ti + (lane >> 4) + 4 * g];
                 crow[g] = M + (size_t)prow[g] * ldm + c0 + (lane & 15);
             }
+            // U12 operand rows: LDS (single-kernel path) or M itself (L2)
+            const double* urow[RB / 4];
+#pragma unroll
+            for (int s = 0; s < RB / 4; ++s)
+                urow[s] = FULL ? stage + (4 * s + (lane >> 4)) * su
+                               : M + (size_t)phys[j0 + 4 * s + (lane >> 4)] * ldm + c0;
             // C operands: from U in the first panel, from M afterwards
             auto cval = [&](int g, int tj) -> double {
                 return j0 == 0 ? u_at(prow[g], c0 + 16 * tj + (lane & 15)) : crow[g][16 * tj];
@@ -464,7 +473,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2))) k_
                 }
 #pragma unroll
                 for (int s = 0; s < RB / 4; ++s)
-                    acc = mfma16(af[s], stage[(4 * s + (lane >> 4)) * su + cc], acc);
+                    acc = mfma16(af[s], urow[s][cc], acc);
 #pragma unroll
                 for (int g = 0; g < 4; ++g) crow[g][16 * tj] = acc[g];
                 acc = nxt;
@@ -877,11 +886,11 @@ static size_t solve_lds_bytes(int ns, int nt) { return (size_t)(nt * 16 + NWAVE 
 
 size_t refresh_ws_doubles_per_job(int ns, int nmax) { return refresh_ws_job(ns, nmax); }
 
-static size_t refresh_lds_bytes(int ns, int nup, int ndn) {
+static size_t refresh_lds_bytes(int ns, int nup, int ndn, bool full = true) {
     const int nmax = nup > ndn ? nup : ndn;
     const int nmin = nup < ndn ? nup : ndn;
     const int su = rup_i(nmax, 16) - RB > 16 ? rup_i(nmax, 16) - RB : 16;
-    const int stage = RB * su > 16 * (rup_i(nmax, 16) | 1) ? RB * su : 16 * (rup_i(nmax, 16) | 1);
+    const int stage = !full ? 0 : RB * su > 16 * (rup_i(nmax, 16) | 1) ? RB * su : 16 * (rup_i(nmax, 16) | 1);
     (void)nmin;
     const size_t dbl = (size_t)(2 * NWAVE * RB + 2 * NWAVE + 2 * RB * LB + stage + nmax + 3 * NWAVE) * sizeof(double);
     const size_t ints = (size_t)(NWAVE + refresh_nrows(ns) + 2 * ns + nmax) * sizeof(int);
@@ -932,19 +941,19 @@ template <typename T>
 static hipError_t launch_refresh_t(const RefreshArgs& a, size_t lds, hipStream_t s) {
     switch (solve_tiles(a.ns, a.nmax)) {
         case 4: {
-            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            hipError_t e = launch_lu<T, 1, false>(a, refresh_lds_bytes(a.ns, a.nup, a.ndn, false), s);
             return e != hipSuccess ? e : launch_solve<T, 4>(a, s);
         }
         case 8: {
-            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            hipError_t e = launch_lu<T, 1, false>(a, refresh_lds_bytes(a.ns, a.nup, a.ndn, false), s);
             return e != hipSuccess ? e : launch_solve<T, 8>(a, s);
         }
         case 14: {
-            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            hipError_t e = launch_lu<T, 1, false>(a, refresh_lds_bytes(a.ns, a.nup, a.ndn, false), s);
             return e != hipSuccess ? e : launch_solve<T, 14>(a, s);
         }
         case 16: {
-            hipError_t e = launch_lu<T, 1, false>(a, lds, s);
+            hipError_t e = launch_lu<T, 1, false>(a, refresh_lds_bytes(a.ns, a.nup, a.ndn, false), s);
             return e != hipSuccess ? e : launch_solve<T, 16>(a, s);
         }
         default:
