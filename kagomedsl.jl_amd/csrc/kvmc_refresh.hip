@@ -868,11 +868,13 @@ k_refresh_solve(RefreshArgs a, int groups) {
     }
     const int s_lo = grp == 0 ? 0 : esite[64 * grp];
     const int s_hi = 64 * (grp + 1) < nE ? esite[64 * (grp + 1)] : ns;
-    const int span = s_hi - s_lo;
-    for (int idx = t; idx < N * span; idx += RT) {
-        const int l = idx / span, s = s_lo + idx % span;
-        const int ks = kap[s];
-        if (ks != 0) put(l, s, (ks - 1 == l) ? 1.0 : 0.0);
+    // lanes over consecutive sites, wavefronts over label columns
+    for (int s0 = s_lo; s0 < s_hi; s0 += 64) {
+        const int s = s0 + lane;
+        const int ks = s < s_hi ? kap[s] : 0;
+        if (ks != 0) {
+            for (int l = wave; l < N; l += NWAVE) put(l, s, (ks - 1 == l) ? 1.0 : 0.0);
+        }
     }
     if (!a.write_w) {
         atomic_max_nonneg2(&a.drift[0], dmax);
