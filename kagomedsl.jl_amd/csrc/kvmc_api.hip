@@ -128,7 +128,8 @@ struct kvmc_handle {
     double* UTdn = nullptr;
     int* kup = nullptr;
     int* kdn = nullptr;
-    void* W = nullptr;
+    void* W = nullptr;              // empty-site rows per spin (w_stride per walker)
+    int* slot = nullptr;            // [nw][2][ns] site -> row of W, -1 when occupied
     void* A = nullptr;
     void* Bm = nullptr;
     int* npend = nullptr;
@@ -165,7 +166,7 @@ struct kvmc_handle {
 namespace {
 
 void free_all(kvmc_handle* h) {
-    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->kup, h->kdn, h->W, h->A,
+    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->kup, h->kdn, h->W, h->slot, h->A,
                     h->Bm, h->npend, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
                     h->tbase, h->rlist, h->rcount, h->failed, h->all_walkers, h->counters,
                     h->logdet, h->eloc, h->drift, h->ws, h->Qup, h->Qdn, h->O};
@@ -197,7 +198,7 @@ SweepArgs sweep_args(kvmc_handle* h) {
     a.ns = h->ns; a.nup = h->nup; a.ndn = h->ndn; a.nb = h->nb;
     a.nocc = std::min(h->nup, h->ndn);
     a.bonds = h->bonds; a.nbr_off = h->nbr_off; a.nbr = h->nbr;
-    a.kup = h->kup; a.kdn = h->kdn; a.W = h->W; a.A = h->A; a.Bm = h->Bm;
+    a.kup = h->kup; a.kdn = h->kdn; a.W = h->W; a.slot = h->slot; a.A = h->A; a.Bm = h->Bm;
     a.npend = h->npend; a.delay = h->delay; a.nmax = h->nmax;
     a.sweeps = h->sweeps; a.stop = h->stop; a.tape_base = h->tbase;
     a.zmu = h->zmu; a.acc_since = h->acc_since;
@@ -213,7 +214,7 @@ RefreshArgs refresh_args(const kvmc_handle* h, int write_w, double pivot_tol) {
     RefreshArgs r{};
     r.ns = h->ns; r.nup = h->nup; r.ndn = h->ndn; r.Uup = h->Uup; r.Udn = h->Udn;
     r.UTup = h->UTup; r.UTdn = h->UTdn;
-    r.kup = h->kup; r.kdn = h->kdn; r.W = h->W; r.A = h->A; r.Bm = h->Bm;
+    r.kup = h->kup; r.kdn = h->kdn; r.W = h->W; r.slot = h->slot; r.A = h->A; r.Bm = h->Bm;
     r.npend = h->npend; r.acc_since = h->acc_since; r.delay = h->delay; r.nmax = h->nmax;
     r.ws = h->ws; r.logdet = h->logdet; r.status = h->status; r.failed = h->failed;
     r.write_w = write_w; r.drift = h->drift; r.pivot_tol = pivot_tol;
@@ -357,9 +358,17 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     ALLOC(h->UTdn, ns * h->ndn);
     ALLOC(h->kup, nw * ns);
     ALLOC(h->kdn, nw * ns);
+    ALLOC(h->slot, nw * 2 * ns);
+    // zero until the first refresh writes it: every entry stays a row of W or -1
+    if (hipMemset(h->slot, 0, nw * 2 * ns * sizeof(int)) != hipSuccess) {
+        free_all(h); delete h; return fail(KVMC_ERR_HIP, "hipMemset of the slot map failed");
+    }
     {
         unsigned char* p = nullptr;
-        ALLOC(p, nw * ns * (h->nup + h->ndn) * h->esize); h->W = p;
+        const size_t wbytes = nw * w_stride(h->ns, h->nup, h->ndn) * h->esize;
+        ALLOC(p, wbytes); h->W = p;
+        // zero once: the padding rows of W must stay zero (no pending term touches them)
+        if (hipMemset(p, 0, wbytes) != hipSuccess) { free_all(h); delete h; return fail(KVMC_ERR_HIP, "hipMemset of W failed"); }
         ALLOC(p, nw * 2 * delay * h->nsp * h->esize); h->A = p;
         ALLOC(p, nw * 2 * delay * h->nmax * h->esize); h->Bm = p;
     }
@@ -664,7 +673,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
 static ElocArgs eloc_args(kvmc_handle* h) {
     ElocArgs e{};
     e.ns = h->ns; e.nup = h->nup; e.ndn = h->ndn; e.nb = h->nb;
-    e.bonds = h->bonds; e.kup = h->kup; e.kdn = h->kdn; e.W = h->W; e.A = h->A; e.Bm = h->Bm;
+    e.bonds = h->bonds; e.kup = h->kup; e.kdn = h->kdn; e.W = h->W; e.slot = h->slot; e.A = h->A; e.Bm = h->Bm;
     e.npend = h->npend; e.delay = h->delay; e.nmax = h->nmax; e.nw = h->nw;
     e.eloc = h->eloc; e.counters = h->counters;
     e.nparams = h->nparams; e.Qup = h->Qup; e.Qdn = h->Qdn; e.O = h->O;
@@ -759,17 +768,28 @@ int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out) {
     SweepArgs a = sweep_args(h);
     a.walker0 = (int)walker;
     HIP_TRY(launch_flush(h->prec, a, 1, s));
-    const size_t ns = h->ns, n = spin ? h->ndn : h->nup;
-    const size_t off = (size_t)walker * ns * (h->nup + h->ndn) + (spin ? ns * h->nup : 0);
+    // W keeps the empty-site rows; the occupied rows are the unit rows e_label
+    const size_t ns = h->ns, n = spin ? h->ndn : h->nup, ld = w_ld(h->ns, (int)n);
+    const size_t off = (size_t)walker * w_stride(h->ns, h->nup, h->ndn) + (spin ? w_off_dn(h->ns, h->nup) : 0);
+    std::vector<double> rows(ld * n);
+    std::vector<int> sl(ns), kap(ns);
     if (h->prec == KVMC_PREC_FP64) {
-        HIP_TRY(hipMemcpyAsync(out, (double*)h->W + off, ns * n * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpyAsync(rows.data(), (double*)h->W + off, ld * n * sizeof(double), hipMemcpyDeviceToHost, s));
     } else {
-        std::vector<float> tmp(ns * n);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)h->W + off, ns * n * sizeof(float), hipMemcpyDeviceToHost, s));
+        std::vector<float> tmp(ld * n);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)h->W + off, ld * n * sizeof(float), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        for (size_t q = 0; q < ns * n; ++q) out[q] = tmp[q];
+        for (size_t q = 0; q < ld * n; ++q) rows[q] = tmp[q];
     }
+    HIP_TRY(hipMemcpyAsync(sl.data(), h->slot + ((size_t)walker * 2 + spin) * ns, ns * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(kap.data(), (spin ? h->kdn : h->kup) + (size_t)walker * ns, ns * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (size_t l = 0; l < n; ++l)
+        for (size_t r = 0; r < ns; ++r)
+            out[l * ns + r] = (sl[r] >= 0 && (size_t)sl[r] < ld) ? rows[l * ld + sl[r]]
+                                                                 : (kap[r] == (int)l + 1 ? 1.0 : 0.0);
     return KVMC_OK;
 }
 

@@ -13,10 +13,12 @@
 //   k_flush    applies pending delayed updates to W.
 //
 // Layout in HBM (per walker, walker-major): kappa_up[ns], kappa_dn[ns]
-// (int32, 1-based labels, 0 = empty); W_up (ns x nup) then W_dn (ns x ndn),
-// column-major, so column l (a label) is contiguous.  For small clusters
-// (both W of a walker within kResidentMaxBytes) the sweep kernel copies W
-// into LDS at launch and back at exit, so sweeps and flushes never touch HBM.
+// (int32, 1-based labels, 0 = empty); W_up (ldw_up x nup) then W_dn (ldw_dn x
+// ndn), the rows of the empty sites only (the occupied rows are unit rows),
+// column-major, so column l (a label) is contiguous; slot[2][ns] maps a site
+// to its row.  For small clusters (both W of a walker within
+// kResidentMaxBytes) the sweep kernel copies W into LDS at launch and back at
+// exit, so sweeps and flushes never touch HBM.
 #include "kvmc_internal.hpp"
 
 #include <float.h>
@@ -38,17 +40,19 @@ template <typename T> constexpr int native_vec() { return 16 / (int)sizeof(T); }
 __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // ---------------------------------------------------------------------------
-// Streaming rank-c update of one spin's W:  W[:, j] += sum_{q<c} A_q * B_q[j].
-// Thread t owns the row vector iv = t % nvec of every column j == lane (mod
-// lanes); its A_q slices stay in registers for the whole pass, B_q[j] is an
-// LDS broadcast.  Requires blockDim.x >= ns / V (checked on the host).
+// Streaming rank-c update of one spin's W (ld rows, column stride ld):
+// W[:, j] += sum_{q<c} A_q * B_q[j].  Thread t owns the row vector
+// iv = t % nvec of every column j == lane (mod lanes); its A_q slices stay in
+// registers for the whole pass, B_q[j] is an LDS broadcast.  Requires
+// blockDim.x >= ld / V (checked on the host).
 // ---------------------------------------------------------------------------
 template <typename T, int V, int DMAX, int UNR = 4>
 __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __restrict__ sA,
-                                              const T* __restrict__ sB, int ns, int nsp,
+                                              const T* __restrict__ sB, int ld, int nsp,
                                               int nmax, int ncols, int c) {
     using VT = typename VecT<T, V>::type;   // UNR: columns in flight per thread
-    const int nvec = ns / V;
+    const int nvec = ld / V;
+    if (nvec == 0) return;
     const int lanes = (int)blockDim.x / nvec;
     const int t = (int)threadIdx.x;
     const int iv = t % nvec, lane = t / nvec;
@@ -62,7 +66,7 @@ __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __res
     for (; j + (UNR - 1) * lanes < ncols; j += UNR * lanes) {
         VT w[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) w[u] = *reinterpret_cast<const VT*>(base + (size_t)(j + u * lanes) * ns);
+        for (int u = 0; u < UNR; ++u) w[u] = *reinterpret_cast<const VT*>(base + (size_t)(j + u * lanes) * ld);
 #pragma unroll
         for (int q = 0; q < DMAX; ++q) {
             if (q < c) {
@@ -71,62 +75,68 @@ __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __res
             }
         }
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) *reinterpret_cast<VT*>(base + (size_t)(j + u * lanes) * ns) = w[u];
+        for (int u = 0; u < UNR; ++u) *reinterpret_cast<VT*>(base + (size_t)(j + u * lanes) * ld) = w[u];
     }
 #pragma nounroll
     for (; j < ncols; j += lanes) {
-        VT w0 = *reinterpret_cast<const VT*>(base + (size_t)j * ns);
+        VT w0 = *reinterpret_cast<const VT*>(base + (size_t)j * ld);
 #pragma unroll
         for (int q = 0; q < DMAX; ++q)
             if (q < c) w0 += a[q] * sB[q * nmax + j];
-        *reinterpret_cast<VT*>(base + (size_t)j * ns) = w0;
+        *reinterpret_cast<VT*>(base + (size_t)j * ld) = w0;
     }
 }
 
-// Current W element: W0[K, l] + sum_{q<c} A_q[K] B_q[l].
+// Current W element of row e (the slot of an empty site) and label l:
+// W0[e, l] + sum_{q<c} A_q[e] B_q[l].
 template <typename T>
-__device__ __forceinline__ T w_cur(const T* __restrict__ W0, const T* sA, const T* sB, int ns,
+__device__ __forceinline__ T w_cur(const T* __restrict__ W0, const T* sA, const T* sB, int ld,
                                    int nsp, int nmax, int K, int l, int c) {
-    T v = W0[(size_t)l * ns + K];
+    T v = W0[(size_t)l * ld + K];
     for (int q = 0; q < c; ++q) v += sA[q * nsp + K] * sB[q * nmax + l];
     return v;
 }
 
-// Appends both spins' rank-1 terms of an accepted move (rows K, labels l,
-// 0-based): A_c = W[:, l], B_c = alpha (W[K, :] - e_l), alpha = -1/W[K, l],
-// exactly the reference's update_W! factors (src/MonteCarlo.jl:279-292).
-// The column and row elements of both W are loaded before any correction or
-// LDS store, so the accept costs one memory round trip.
+// Appends both spins' rank-1 terms of an accepted move (target rows K, i.e.
+// the slots of the sites the electrons move to, labels l, 0-based):
+// B_c = alpha (W[K, :] - e_l), alpha = -1/W[K, l], and A_c = W[:, l] + e_K,
+// the reference's update_W! factors (src/MonteCarlo.jl:279-292) on the
+// empty-site rows.  The + e_K turns row K, which now stands for the site the
+// electron left (unit row e_l before the move), into that site's new row
+// e_l + B_c.  The column and row elements of both W are loaded before any
+// correction or LDS store, so the accept costs one memory round trip.
 template <typename T>
 __device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T* __restrict__ Wdn,
-                                             T* sAu, T* sBu, T* sAd, T* sBd, int ns, int nsp,
+                                             T* sAu, T* sBu, T* sAd, T* sBd, int ldu, int ldd, int nsp,
                                              int nmax, int nup, int ndn, int Kup, int lup, int Kdn,
                                              int ldn, int c, T wu, T wd) {
     constexpr int CH = 4;
     const int bd = (int)blockDim.x, t0 = (int)threadIdx.x;
-    const int nmx = ns > nmax ? ns : nmax;
+    const int nld = ldu > ldd ? ldu : ldd;
+    const int nmx = nld > nmax ? nld : nmax;
     for (int base = 0; base < nmx; base += CH * bd) {
         T cu[CH], cd[CH], ru[CH], rd[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const int t = base + t0 + u * bd;
-            cu[u] = t < ns ? Wup[(size_t)lup * ns + t] : T(0);
-            cd[u] = t < ns ? Wdn[(size_t)ldn * ns + t] : T(0);
-            ru[u] = t < nup ? Wup[(size_t)t * ns + Kup] : T(0);
-            rd[u] = t < ndn ? Wdn[(size_t)t * ns + Kdn] : T(0);
+            cu[u] = t < ldu ? Wup[(size_t)lup * ldu + t] : T(0);
+            cd[u] = t < ldd ? Wdn[(size_t)ldn * ldd + t] : T(0);
+            ru[u] = t < nup ? Wup[(size_t)t * ldu + Kup] : T(0);
+            rd[u] = t < ndn ? Wdn[(size_t)t * ldd + Kdn] : T(0);
         }
         const T au = T(-1) / wu, ad = T(-1) / wd;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const int t = base + t0 + u * bd;
-            if (t < ns) {
-                T vu = cu[u], vd = cd[u];
-                for (int q = 0; q < c; ++q) {
-                    vu += sAu[q * nsp + t] * sBu[q * nmax + lup];
-                    vd += sAd[q * nsp + t] * sBd[q * nmax + ldn];
-                }
-                sAu[c * nsp + t] = vu;
-                sAd[c * nsp + t] = vd;
+            if (t < ldu) {
+                T vu = cu[u];
+                for (int q = 0; q < c; ++q) vu += sAu[q * nsp + t] * sBu[q * nmax + lup];
+                sAu[c * nsp + t] = vu + (t == Kup ? T(1) : T(0));
+            }
+            if (t < ldd) {
+                T vd = cd[u];
+                for (int q = 0; q < c; ++q) vd += sAd[q * nsp + t] * sBd[q * nmax + ldn];
+                sAd[c * nsp + t] = vd + (t == Kdn ? T(1) : T(0));
             }
             if (t < nup) {
                 T v = ru[u];
@@ -191,7 +201,7 @@ __device__ inline double block_sum_double(double v, double* red) {
 // ---------------------------------------------------------------------------
 __host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay, int nb) {
     const size_t nsp = (size_t)(ns + 3) / 4 * 4;
-    const size_t b = 2 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
+    const size_t b = 4 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
                      (size_t)nb * sizeof(uint32_t) + (size_t)(ns + 1 + 2 * nb) * sizeof(uint16_t);
     return (b + 15) / 16 * 16;
 }
@@ -205,18 +215,22 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     const int delay = a.delay;
     int* s_kup = reinterpret_cast<int*>(lds);
     int* s_kdn = s_kup + nsp;
-    T* s_A = reinterpret_cast<T*>(s_kdn + nsp);           // [2][delay][nsp]
+    int* s_slu = s_kdn + nsp;                             // site -> row of W_up (-1: occupied)
+    int* s_sld = s_slu + nsp;
+    T* s_A = reinterpret_cast<T*>(s_sld + nsp);           // [2][delay][nsp]
     T* s_B = s_A + 2 * delay * nsp;                       // [2][delay][nmax]
     double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
     uint32_t* s_bond = reinterpret_cast<uint32_t*>(s_red + 16);   // [nb] packed bond table
     uint16_t* s_noff = reinterpret_cast<uint16_t*>(s_bond + nb);    // [ns + 1]
     uint16_t* s_nbr = s_noff + ns + 1;                              // [<= 2 nb]
 
-    const size_t wstride = (size_t)ns * (nup + ndn);
+    const size_t wstride = w_stride(ns, nup, ndn);
+    const int ldu = w_ld(ns, nup), ldd = w_ld(ns, ndn);
     T* const Wg = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
     // W of this walker: in HBM, or (RES) an LDS copy for the whole launch
     T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nmax, delay, nb)) : Wg;
-    T* Wdn = Wup + (size_t)ns * nup;
+    T* Wdn = Wup + w_off_dn(ns, nup);
+    int* const slot_g = a.slot + (size_t)w * 2 * ns;
     T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsp;
     T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
     T* sAu = s_A;
@@ -233,6 +247,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
         s_kup[t] = a.kup[(size_t)w * ns + t];
         s_kdn[t] = a.kdn[(size_t)w * ns + t];
+        s_slu[t] = slot_g[t];
+        s_sld[t] = slot_g[ns + t];
     }
     if (c > 0) {
         for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
@@ -290,7 +306,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         const double zr = (double)zmu / zmax;
         int oc = OUT_PRE_REJECT;
         bool mv = false, acc = false;
-        int i = 0, site = 0, lup = 0, ldn = 0, Kup = 0, Kdn = 0, dz = 0;
+        int i = 0, site = 0, lup = 0, ldn = 0, Kup = 0, Kdn = 0, dz = 0;   // K: target sites
         bool can1 = false;
         T wu = T(0), wd = T(0);
         if (valid && !(a.kernel_mode == KERNEL_REFERENCE && r > zr)) {
@@ -309,8 +325,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
                 ldn = (can1 ? sd : id) - 1;
                 Kup = can1 ? site : i;
                 Kdn = can1 ? i : site;
-                wu = w_cur(Wup, sAu, sBu, ns, nsp, nmax, Kup, lup, c);
-                wd = w_cur(Wdn, sAd, sBd, ns, nsp, nmax, Kdn, ldn, c);
+                // targets are empty for their spin; max() keeps a walker with an
+                // inconsistent state (failed refresh) inside W
+                wu = w_cur(Wup, sAu, sBu, ldu, nsp, nmax, max(s_slu[Kup], 0), lup, c);
+                wd = w_cur(Wdn, sAd, sBd, ldd, nsp, nmax, max(s_sld[Kdn], 0), ldn, c);
                 dz = z_delta(s_bond, s_noff, s_nbr, s_kup, s_kdn, i, site);
                 const T ratio = wu * wd;
                 const double acc_prob = (double)ratio * (double)ratio;
@@ -345,10 +363,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
             const int tdz = __shfl(dz, tl, 64);
             ++n_acc;
             ++acc_since;
-            append_terms(Wup, Wdn, sAu, sBu, sAd, sBd, ns, nsp, nmax, nup, ndn, tKup, tlup, tKdn,
+            const int eu = max(s_slu[tKup], 0), ed = max(s_sld[tKdn], 0);   // rows of the target sites
+            append_terms(Wup, Wdn, sAu, sBu, sAd, sBd, ldu, ldd, nsp, nmax, nup, ndn, eu, tlup, ed,
                          tldn, c, twu, twd);
             __syncthreads();
             if (threadIdx.x == 0) {
+                // each electron's row goes to the site it left
+                const int fu = tcan1 ? ti : tsite, fd = tcan1 ? tsite : ti;   // source sites
+                s_slu[fu] = eu; s_slu[tKup] = -1;
+                s_sld[fd] = ed; s_sld[tKdn] = -1;
                 if (tcan1) {
                     s_kup[ti] = 0; s_kup[tsite] = tlup + 1;
                     s_kdn[ti] = tldn + 1; s_kdn[tsite] = 0;
@@ -359,8 +382,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
             }
             ++c;
             if (c == delay) {
-                stream_update<T, V, DMAX>(Wup, sAu, sBu, ns, nsp, nmax, nup, c);
-                stream_update<T, V, DMAX>(Wdn, sAd, sBd, ns, nsp, nmax, ndn, c);
+                stream_update<T, V, DMAX>(Wup, sAu, sBu, ldu, nsp, nmax, nup, c);
+                stream_update<T, V, DMAX>(Wdn, sAd, sBd, ldd, nsp, nmax, ndn, c);
                 c = 0;
                 ++n_flush;
             }
@@ -378,6 +401,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
         a.kup[(size_t)w * ns + t] = s_kup[t];
         a.kdn[(size_t)w * ns + t] = s_kdn[t];
+        slot_g[t] = s_slu[t];
+        slot_g[ns + t] = s_sld[t];
     }
     if (RES) {
         for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wg[t] = Wup[t];
@@ -407,14 +432,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
 
 bool sweep_resident(int prec, int ns, int nup, int ndn) {
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
-    return (size_t)ns * (nup + ndn) * es <= kResidentMaxBytes;
+    return w_stride(ns, nup, ndn) * es <= kResidentMaxBytes;
 }
 
 size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay, int nb) {
     const int nmax = nup > ndn ? nup : ndn;
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
     return sweep_lds_base(es, ns, nmax, delay, nb) +
-           (sweep_resident(prec, ns, nup, ndn) ? (size_t)ns * (nup + ndn) * es : 0);
+           (sweep_resident(prec, ns, nup, ndn) ? w_stride(ns, nup, ndn) * es : 0);
 }
 
 template <typename T, int V, int D>
@@ -443,11 +468,9 @@ static hipError_t launch_sweep_v(const SweepArgs& a, int nw, int threads, size_t
 
 hipError_t launch_sweep(int prec, const SweepArgs& a, int nw, int threads, size_t lds,
                         hipStream_t s) {
-    if (prec == 0)
-        return (a.ns % 2 == 0) ? launch_sweep_v<double, 2>(a, nw, threads, lds, s)
-                               : launch_sweep_v<double, 1>(a, nw, threads, lds, s);
-    return (a.ns % 4 == 0) ? launch_sweep_v<float, 4>(a, nw, threads, lds, s)
-                           : launch_sweep_v<float, 1>(a, nw, threads, lds, s);
+    // rows of W are padded to a multiple of 4: always whole 16-byte vectors
+    if (prec == 0) return launch_sweep_v<double, 2>(a, nw, threads, lds, s);
+    return launch_sweep_v<float, 4>(a, nw, threads, lds, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -467,12 +490,11 @@ __global__ void __launch_bounds__(1024) k_flush(SweepArgs a) {
     for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
     for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
     __syncthreads();
-    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
-    T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
-    T* Wdn = Wup + (size_t)ns * a.nup;
-    stream_update<T, V, kMaxDelay, 2>(Wup, s_A, s_B, ns, nsp, nmax, a.nup, c);
-    stream_update<T, V, kMaxDelay, 2>(Wdn, s_A + delay * nsp, s_B + delay * nmax, ns, nsp, nmax,
-                                      a.ndn, c);
+    T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
+    T* Wdn = Wup + w_off_dn(ns, a.nup);
+    stream_update<T, V, kMaxDelay, 2>(Wup, s_A, s_B, w_ld(ns, a.nup), nsp, nmax, a.nup, c);
+    stream_update<T, V, kMaxDelay, 2>(Wdn, s_A + delay * nsp, s_B + delay * nmax, w_ld(ns, a.ndn), nsp,
+                                      nmax, a.ndn, c);
     __syncthreads();
     if (threadIdx.x == 0) a.npend[w] = 0;
 }
@@ -483,24 +505,18 @@ hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
     const size_t lds = (size_t)2 * a.delay * (nsp + a.nmax) * es;
     const int V = prec == 0 ? 2 : 4;
     int threads = 256;
-    const int nvec = a.ns % V == 0 ? a.ns / V : a.ns;
+    const int ldm = w_ld(a.ns, a.nup) > w_ld(a.ns, a.ndn) ? w_ld(a.ns, a.nup) : w_ld(a.ns, a.ndn);
+    const int nvec = ldm / V;
     while (threads < nvec) threads += 64;
     dim3 grid(nw), block(threads);
     if (lds > 64 * 1024) {
-        const void* fn = prec == 0 ? (a.ns % 2 == 0 ? reinterpret_cast<const void*>(&k_flush<double, 2>)
-                                                    : reinterpret_cast<const void*>(&k_flush<double, 1>))
-                                   : (a.ns % 4 == 0 ? reinterpret_cast<const void*>(&k_flush<float, 4>)
-                                                    : reinterpret_cast<const void*>(&k_flush<float, 1>));
+        const void* fn = prec == 0 ? reinterpret_cast<const void*>(&k_flush<double, 2>)
+                                   : reinterpret_cast<const void*>(&k_flush<float, 4>);
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    if (prec == 0) {
-        if (a.ns % 2 == 0) hipLaunchKernelGGL((k_flush<double, 2>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_flush<double, 1>), grid, block, lds, s, a);
-    } else {
-        if (a.ns % 4 == 0) hipLaunchKernelGGL((k_flush<float, 4>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_flush<float, 1>), grid, block, lds, s, a);
-    }
+    if (prec == 0) hipLaunchKernelGGL((k_flush<double, 2>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_flush<float, 4>), grid, block, lds, s, a);
     return hipGetLastError();
 }
 
@@ -510,9 +526,9 @@ hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
 // stride over the bond list, FP64 accumulation, shuffle reduction.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ns, int nsp,
+__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ld, int nsp,
                                          int nmax, int K, int l, int c) {
-    double v = (double)W0[(size_t)l * ns + K];
+    double v = (double)W0[(size_t)l * ld + K];
     for (int q = 0; q < c; ++q) v += (double)(gA[q * nsp + K] * gB[q * nmax + l]);
     return v;
 }
@@ -523,11 +539,13 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
     const int w = (int)blockIdx.x * ((int)blockDim.x / 64) + wave;
     if (w >= a.nw) return;
     const int ns = a.ns, nmax = a.nmax, nsp = round_up(ns, 4);
-    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
-    const T* Wup = reinterpret_cast<const T*>(a.W) + (size_t)w * wstride;
-    const T* Wdn = Wup + (size_t)ns * a.nup;
+    const int ldu = w_ld(ns, a.nup), ldd = w_ld(ns, a.ndn);
+    const T* Wup = reinterpret_cast<const T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
+    const T* Wdn = Wup + w_off_dn(ns, a.nup);
     const int* ku = a.kup + (size_t)w * ns;
     const int* kd = a.kdn + (size_t)w * ns;
+    const int* slu = a.slot + (size_t)w * 2 * ns;     // rows of the (spin-)empty sites
+    const int* sld = slu + ns;
     const int c = a.npend[w];
     const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * a.delay * nsp;
     const T* gAd = gAu + (size_t)a.delay * nsp;
@@ -541,12 +559,13 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
         const int iu = ku[i], ju = ku[j], id = kd[i], jd = kd[j];
         const bool t1 = ju != 0 && id != 0;   // S+_i S-_j
         const bool t2 = iu != 0 && jd != 0;   // S-_i S+_j
+        // the target sites are empty for the moving spin (no site holds both)
         if (t1)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ns, nsp, nmax, i, ju - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ns, nsp, nmax, j, id - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(slu[i], 0), ju - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sld[j], 0), id - 1, c);
         if (t2)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ns, nsp, nmax, j, iu - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ns, nsp, nmax, i, jd - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(slu[j], 0), iu - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sld[i], 0), jd - 1, c);
         nap += (iu != 0 && jd != 0) || (ju != 0 && id != 0);
         const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
         diag += szi * szj;
@@ -574,6 +593,8 @@ hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s) {
 // eigenvectors of a real symmetric H), tildeU^-1 = U^T W, so
 //   O_k = tr(tildeU^-1 d tildeU) = sum_l sum_r W[r, l] Q_k[R_l, r],
 //   Q_k = dU_k U^T (ns x ns, built once per ansatz).
+// Occupied rows of W are unit rows, so the occupied sites contribute
+// Q_k[R_l, R_l] and the empty ones W[slot(r), l] Q_k[R_l, r].
 // One workgroup per walker; wavefronts take labels, lanes stride over r.
 // Requires W without pending terms (the host flushes first).
 // ---------------------------------------------------------------------------
@@ -592,19 +613,26 @@ __global__ void __launch_bounds__(256) k_grad(ElocArgs a) {
     }
     for (int p = tid; p < P; p += (int)blockDim.x) acc[p] = 0.0;
     __syncthreads();
-    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
     const int wave = tid / 64, lane = tid % 64, nwave = (int)blockDim.x / 64;
     for (int spin = 0; spin < 2; ++spin) {
         const int N = spin ? a.ndn : a.nup;
-        const T* W = reinterpret_cast<const T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+        const int ld = w_ld(ns, N);
+        const T* W = reinterpret_cast<const T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn) +
+                     (spin ? w_off_dn(ns, a.nup) : 0);
+        const int* sl = a.slot + ((size_t)w * 2 + spin) * ns;
         const double* Q = spin ? a.Qdn : a.Qup;
         for (int p = 0; p < P; ++p) {
             const double* Qp = Q + (size_t)p * ns * ns;
             double part = 0.0;
             for (int l = wave; l < N; l += nwave) {
-                const double* qrow = Qp + (size_t)Rsite[spin * nmax + l] * ns;
-                const T* wcol = W + (size_t)l * ns;
-                for (int r = lane; r < ns; r += 64) part += (double)wcol[r] * qrow[r];
+                const int R = Rsite[spin * nmax + l];
+                const double* qrow = Qp + (size_t)R * ns;
+                const T* wcol = W + (size_t)l * ld;
+                if (lane == 0) part += qrow[R];
+                for (int r = lane; r < ns; r += 64) {
+                    const int e = sl[r];
+                    if (e >= 0) part += (double)wcol[e] * qrow[r];
+                }
             }
             for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
             if (lane == 0) atomicAdd(&acc[p], part);

@@ -19,6 +19,20 @@ enum : int { CNT_MOVES = 0, CNT_ACCEPTED = 1, CNT_FLUSHES = 2, CNT_SWEEPS = 3, C
 // Largest number of pending rank-1 updates kept per spin before a flush.
 constexpr int kMaxDelay = 16;
 
+// W of one spin keeps only the rows of the empty sites ("slots"): the row of
+// an occupied site is the unit row e_label by construction (W = U tildeU^-1,
+// W[R_l, :] = e_l).  Per walker: W_up (ldw_up x nup) then W_dn (ldw_dn x
+// ndn), column-major, ldw = rup4(ns - N); rows ns - N .. ldw - 1 are zero
+// padding.  slot[w][spin][site] is the row of an empty site, -1 for an
+// occupied one.  After a refresh the slots follow the empty sites in site
+// order; an accepted move hands the slot of the newly occupied site to the
+// site the electron left.
+__host__ __device__ inline int w_ld(int ns, int n) { return (ns - n + 3) / 4 * 4; }
+__host__ __device__ inline size_t w_off_dn(int ns, int nup) { return (size_t)w_ld(ns, nup) * nup; }
+__host__ __device__ inline size_t w_stride(int ns, int nup, int ndn) {
+    return w_off_dn(ns, nup) + (size_t)w_ld(ns, ndn) * ndn;
+}
+
 // Philox4x32-10 (Salmon et al., SC'11): counter-based, so every
 // (walker, sweep) draw is addressable without carried state.
 __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -55,8 +69,9 @@ struct SweepArgs {
     const int* nbr;                 // bond ids touching each site
     int* kup;                       // [nw][ns] labels 1-based, 0 empty
     int* kdn;
-    void* W;                        // [nw][ns*(nup+ndn)] column-major per spin
-    void* A;                        // [nw][2][delay][ns]  pending columns
+    void* W;                        // [nw][w_stride] empty-site rows, column-major per spin
+    int* slot;                      // [nw][2][ns] site -> row of W (-1: occupied)
+    void* A;                        // [nw][2][delay][nsp]  pending columns (over rows of W)
     void* Bm;                       // [nw][2][delay][nmax] pending scaled rows
     int* npend;                     // [nw] pending count
     int delay;                      // 1 = immediate rank-1 update
@@ -86,7 +101,8 @@ struct RefreshArgs {
     const double* UTdn;
     const int* kup;                 // [nw][ns]
     const int* kdn;
-    void* W;                        // [nw][ns*(nup+ndn)]
+    void* W;                        // [nw][w_stride]
+    int* slot;                      // [nw][2][ns] rewritten with W (write_w), read in drift mode
     const void* A;                  // pending (drift mode reads W + A B^T)
     const void* Bm;
     int* npend;                     // cleared for refreshed walkers (write_w)
@@ -111,6 +127,7 @@ struct ElocArgs {
     const int* kup;
     const int* kdn;
     const void* W;
+    const int* slot;
     const void* A;
     const void* Bm;
     const int* npend;

@@ -593,12 +593,17 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     }
 
     // ---- 3. W from X, label column by label column ------------------------
-    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
-    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+    // Empty site s takes row rowof[s] - N (site order); occupied rows are unit
+    // rows and are not stored.  Drift mode reads the current rows through the
+    // current slot map.
+    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn) +
+                         (spin ? w_off_dn(ns, a.nup) : 0);
+    const int ld = w_ld(ns, N);
+    int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * ns;
     // Blocks of SB consecutive sites: the solved rows of the block's empty
     // sites are staged in LDS (contiguous row reads, odd row stride against
-    // bank conflicts), then every label column of W gets its SB entries of the
-    // block written as one contiguous run.
+    // bank conflicts), then every label column of W gets the block's entries
+    // written as one contiguous run.
     constexpr int SB = 16;
     const int xs = (nc16 | 1);                       // LDS row stride (odd)
     double* xb = stage;                              // [SB][xs]
@@ -606,7 +611,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     const int cpend = a.write_w ? 0 : a.npend[w];
     const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
-    double dmax = 0.0, vmax = 0.0;
+    double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     for (int sb = 0; sb < ns; sb += SB) {
         for (int i = wave; i < SB; i += NWAVE) {
             const int s = sb + i;
@@ -618,22 +623,24 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         __syncthreads();
         for (int idx = t; idx < N * SB; idx += RT) {
             const int l = idx / SB, i = idx % SB, s = sb + i;
-            if (s >= ns) continue;
-            const int ks = kap[s];
-            const double v = ks != 0 ? ((ks - 1 == l) ? 1.0 : 0.0) : xb[i * xs + pinv[l]];
-            T* __restrict__ dst = Wo + (size_t)l * ns + s;
+            if (s >= ns || kap[s] != 0) continue;
+            const double v = xb[i * xs + pinv[l]];
+            const int row = a.write_w ? rowof[s] - N : max(sl[s], 0);
+            T* __restrict__ dst = Wo + (size_t)l * ld + row;
             if (a.write_w) {
                 *dst = (T)v;
             } else {
                 double cur = (double)*dst;
-                for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + s] * (double)gB[c * nmax + l];
+                for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + row] * (double)gB[c * nmax + l];
                 dmax = fmax(dmax, fabs(v - cur));
                 vmax = fmax(vmax, fabs(v));
             }
         }
         __syncthreads();
     }
-    if (!a.write_w) {
+    if (a.write_w) {
+        for (int s = t; s < ns; s += RT) sl[s] = kap[s] == 0 ? rowof[s] - N : -1;
+    } else {
         atomic_max_nonneg2(&a.drift[0], dmax);
         atomic_max_nonneg2(&a.drift[1], vmax);
     }
@@ -837,46 +844,49 @@ k_refresh_solve(RefreshArgs a, int groups) {
         }
     }
 
-    // ---- W: the solved tiles, then the unit rows of the occupied sites ----
-    const size_t wstride = (size_t)ns * (a.nup + a.ndn);
-    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * wstride + (spin ? (size_t)ns * a.nup : 0);
+    // ---- W: the solved tiles, one row per empty site; the slot map --------
+    // Write mode: the empty sites take rows 0..nE-1 in site order and the
+    // map is rewritten for this workgroup's span of sites.  Drift mode reads
+    // the current rows through the current map.
+    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn) +
+                         (spin ? w_off_dn(ns, a.nup) : 0);
+    const int ld = w_ld(ns, N);
+    int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * ns;
     const int nsp = rup_i(ns, 4);
     const int cpend = a.write_w ? 0 : a.npend[w];
     const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
-    double dmax = 0.0, vmax = 0.0;
-    auto put = [&](int l, int s, double v) {
-        T* __restrict__ dst = Wo + (size_t)l * ns + s;
-        if (a.write_w) {
-            *dst = (T)v;
-        } else {
-            double cur = (double)*dst;
-            for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + s] * (double)gB[c * nmax + l];
-            dmax = fmax(dmax, fabs(v - cur));
-            vmax = fmax(vmax, fabs(v));
-        }
-    };
+    double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     if (site >= 0) {
+        const int row = a.write_w ? e : max(sl[site], 0);
 #pragma unroll
         for (int J = 0; J < NT; ++J) {
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int c = 16 * J + kr + 4 * g;
-                if (c < N) put(lab[c], site, tile[J][g]);
+                if (c < N) {
+                    const int l = lab[c];
+                    const double v = tile[J][g];
+                    T* __restrict__ dst = Wo + (size_t)l * ld + row;
+                    if (a.write_w) {
+                        *dst = (T)v;
+                    } else {
+                        double cur = (double)*dst;
+                        for (int q = 0; q < cpend; ++q) cur += (double)gA[q * nsp + row] * (double)gB[q * nmax + l];
+                        dmax = fmax(dmax, fabs(v - cur));
+                        vmax = fmax(vmax, fabs(v));
+                    }
+                }
             }
         }
     }
-    const int s_lo = grp == 0 ? 0 : esite[64 * grp];
-    const int s_hi = 64 * (grp + 1) < nE ? esite[64 * (grp + 1)] : ns;
-    // lanes over consecutive sites, wavefronts over label columns
-    for (int s0 = s_lo; s0 < s_hi; s0 += 64) {
-        const int s = s0 + lane;
-        const int ks = s < s_hi ? kap[s] : 0;
-        if (ks != 0) {
-            for (int l = wave; l < N; l += NWAVE) put(l, s, (ks - 1 == l) ? 1.0 : 0.0);
-        }
-    }
-    if (!a.write_w) {
+    if (a.write_w) {
+        const int s_lo = grp == 0 ? 0 : esite[64 * grp];
+        const int s_hi = 64 * (grp + 1) < nE ? esite[64 * (grp + 1)] : ns;
+        for (int s = s_lo + t; s < s_hi; s += RT)
+            if (kap[s] != 0) sl[s] = -1;
+        if (t < 64 && 64 * grp + t < nE) sl[esite[64 * grp + t]] = 64 * grp + t;
+    } else {
         atomic_max_nonneg2(&a.drift[0], dmax);
         atomic_max_nonneg2(&a.drift[1], vmax);
     }
