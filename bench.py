@@ -42,9 +42,9 @@ FP64_PEAK_TFLOPS = 78.6
 # walker refreshed by the kernel pair k_refresh_blocked + k_refresh_solve, from
 # rocprofv3 --pmc passes over this script's default FP32 run (summaries in
 # profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 6_422_366}
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_130_244}
 # the same for k_sweep, per walker-sweep (FP32, delay 8, L=12)
-SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 33_905}
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 20_252}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),
@@ -66,14 +66,17 @@ def sync():
 
 
 def sweep_bytes(cnt, s, ns, nup, ndn, nw):
-    """Algorithmic HBM bytes of the sweep kernel for one kvmc_sweep call.
+    """Algorithmic HBM bytes of the sweep kernel for one kvmc_sweep call.  W
+    holds the rows of the empty sites only (ns - N per spin; the occupied
+    rows are unit rows), plus a site -> row map per spin.
 
-    flush:    read + write both W once          2 s ns (nup + ndn)
-    accepted: column l + row K of both W        s (2 ns + nup + ndn)
+    flush:    read + write both W once          2 s ((ns - nup) nup + (ns - ndn) ndn)
+    accepted: column l + row K of both W        s ((ns - nup) + (ns - ndn) + nup + ndn) = 2 s ns
     move:     two W gathers for the ratio       2 s
-    launch:   kappa load + store per walker     4 * 4 ns"""
-    return (cnt["flushes"] * 2 * s * ns * (nup + ndn) + cnt["accepted"] * s * (2 * ns + nup + ndn)
-            + cnt["moves"] * 2 * s + cnt["launches"] * nw * 16 * ns)
+    launch:   kappa and row maps, load + store  2 * 2 * 2 * 4 ns"""
+    rows = (ns - nup) * nup + (ns - ndn) * ndn
+    return (cnt["flushes"] * 2 * s * rows + cnt["accepted"] * 2 * s * ns
+            + cnt["moves"] * 2 * s + cnt["launches"] * nw * 32 * ns)
 
 
 def refresh_flops(ns, n):
@@ -86,8 +89,9 @@ def refresh_flops(ns, n):
 
 def refresh_min_bytes(ns, n, s):
     """Algorithmic HBM bytes of one spin's refresh: read U (FP64, ns x n) and
-    kappa, write W (s bytes per element)."""
-    return 8 * ns * n + 4 * ns + s * ns * n
+    kappa, write the ns - n empty-site rows of W (s bytes per element) and
+    the site -> row map."""
+    return 8 * ns * n + 4 * ns + s * (ns - n) * n + 4 * ns
 
 
 def run_form(nat, ham, args, precision, delay, k, label):
