@@ -309,8 +309,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     *out = nullptr;
     if (d->ns <= 0 || d->n_up <= 0 || d->n_down <= 0 || d->n_up + d->n_down != d->ns)
         return fail(KVMC_ERR_INVALID, "need n_up + n_down == ns (one spinon per site)");
-    if (d->ns > 65535 || d->nb > 32767)
-        return fail(KVMC_ERR_INVALID, "more than 65535 sites or 32767 bonds (16-bit bond tables)");
+    if (d->ns > 32767 || d->nb > 32767)
+        return fail(KVMC_ERR_INVALID, "more than 32767 sites or bonds (16-bit site, label and bond tables)");
     if (d->nb <= 0 || !d->bonds || !d->u_up || !d->u_down)
         return fail(KVMC_ERR_INVALID, "bonds and orbitals are required");
     if (d->nwalkers <= 0 || d->nwalkers > INT_MAX / 2)

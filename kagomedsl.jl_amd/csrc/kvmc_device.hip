@@ -163,7 +163,7 @@ __device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T*
 // sites bonds[b] & 0xffff and bonds[b] >> 16; nbr[nbr_off[x] .. nbr_off[x+1])
 // lists the bonds touching site x.
 __device__ __forceinline__ int z_delta(const uint32_t* bonds, const uint16_t* nbr_off,
-                                       const uint16_t* nbr, const int* s_kup, const int* s_kdn,
+                                       const uint16_t* nbr, const int16_t* s_kup, const int16_t* s_kdn,
                                        int i, int j) {
     int flip = 0, ap = 0;
 #pragma unroll
@@ -201,7 +201,7 @@ __device__ inline double block_sum_double(double v, double* red) {
 // ---------------------------------------------------------------------------
 __host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay, int nb) {
     const size_t nsp = (size_t)(ns + 3) / 4 * 4;
-    const size_t b = 4 * nsp * sizeof(int) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
+    const size_t b = 4 * nsp * sizeof(int16_t) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
                      (size_t)nb * sizeof(uint32_t) + (size_t)(ns + 1 + 2 * nb) * sizeof(uint16_t);
     return (b + 15) / 16 * 16;
 }
@@ -213,11 +213,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     const int ns = a.ns, nup = a.nup, ndn = a.ndn, nb = a.nb, nmax = a.nmax;
     const int nsp = round_up(ns, 4);
     const int delay = a.delay;
-    int* s_kup = reinterpret_cast<int*>(lds);
-    int* s_kdn = s_kup + nsp;
-    int* s_slu = s_kdn + nsp;                             // site -> row of W_up (-1: occupied)
-    int* s_sld = s_slu + nsp;
-    T* s_A = reinterpret_cast<T*>(s_sld + nsp);           // [2][delay][nsp]
+    // labels and rows fit 16 bits (ns <= 65535 is checked on the host)
+    int16_t* s_kup = reinterpret_cast<int16_t*>(lds);
+    int16_t* s_kdn = s_kup + nsp;
+    int16_t* s_slu = s_kdn + nsp;                         // site -> row of W_up (-1: occupied)
+    int16_t* s_sld = s_slu + nsp;
+    T* s_A = reinterpret_cast<T*>(s_sld + nsp);           // [2][delay][nsp] (8 nsp bytes: 16-byte aligned)
     T* s_B = s_A + 2 * delay * nsp;                       // [2][delay][nmax]
     double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
     uint32_t* s_bond = reinterpret_cast<uint32_t*>(s_red + 16);   // [nb] packed bond table
@@ -245,10 +246,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     int acc_since = a.acc_since[w];
 
     for (int t = (int)threadIdx.x; t < ns; t += (int)blockDim.x) {
-        s_kup[t] = a.kup[(size_t)w * ns + t];
-        s_kdn[t] = a.kdn[(size_t)w * ns + t];
-        s_slu[t] = slot_g[t];
-        s_sld[t] = slot_g[ns + t];
+        s_kup[t] = (int16_t)a.kup[(size_t)w * ns + t];
+        s_kdn[t] = (int16_t)a.kdn[(size_t)w * ns + t];
+        s_slu[t] = (int16_t)slot_g[t];
+        s_sld[t] = (int16_t)slot_g[ns + t];
     }
     if (c > 0) {
         for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
@@ -370,14 +371,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
             if (threadIdx.x == 0) {
                 // each electron's row goes to the site it left
                 const int fu = tcan1 ? ti : tsite, fd = tcan1 ? tsite : ti;   // source sites
-                s_slu[fu] = eu; s_slu[tKup] = -1;
-                s_sld[fd] = ed; s_sld[tKdn] = -1;
+                s_slu[fu] = (int16_t)eu; s_slu[tKup] = -1;
+                s_sld[fd] = (int16_t)ed; s_sld[tKdn] = -1;
                 if (tcan1) {
-                    s_kup[ti] = 0; s_kup[tsite] = tlup + 1;
-                    s_kdn[ti] = tldn + 1; s_kdn[tsite] = 0;
+                    s_kup[ti] = 0; s_kup[tsite] = (int16_t)(tlup + 1);
+                    s_kdn[ti] = (int16_t)(tldn + 1); s_kdn[tsite] = 0;
                 } else {
-                    s_kup[ti] = tlup + 1; s_kup[tsite] = 0;
-                    s_kdn[ti] = 0; s_kdn[tsite] = tldn + 1;
+                    s_kup[ti] = (int16_t)(tlup + 1); s_kup[tsite] = 0;
+                    s_kdn[ti] = 0; s_kdn[tsite] = (int16_t)(tldn + 1);
                 }
             }
             ++c;
