@@ -157,7 +157,9 @@ int kvmc_measure_grad(kvmc_handle* h, double* eloc, double* o);
 int kvmc_get_logdet(kvmc_handle* h, double* logdet);
 
 /* Copies W_sigma (spin 0 = up, 1 = down) of one walker as FP64, column-major
- * ns x N, with pending delayed updates applied. */
+ * ns x N, with pending delayed updates applied.  The device keeps only the
+ * rows of the empty sites; the rows of occupied sites are returned as the
+ * unit rows e_label they are by construction. */
 int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out);
 
 /* Sets / reads ctx.sweeps of every walker (Carlo's sweep counter). */
