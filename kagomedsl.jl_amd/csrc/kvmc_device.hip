@@ -534,19 +534,42 @@ __device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, 
     return v;
 }
 
-template <typename T>
+// STAGE: each wavefront first copies its walker's labels and row maps into
+// LDS (16-bit, coalesced), so the bond loop goes to memory only for W.
+template <typename T, bool STAGE>
 __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
     const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
     const int w = (int)blockIdx.x * ((int)blockDim.x / 64) + wave;
-    if (w >= a.nw) return;
+    if (w >= a.nw) return;   // no block-wide barrier below: each wavefront works alone
     const int ns = a.ns, nmax = a.nmax, nsp = round_up(ns, 4);
     const int ldu = w_ld(ns, a.nup), ldd = w_ld(ns, a.ndn);
     const T* Wup = reinterpret_cast<const T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
     const T* Wdn = Wup + w_off_dn(ns, a.nup);
-    const int* ku = a.kup + (size_t)w * ns;
-    const int* kd = a.kdn + (size_t)w * ns;
-    const int* slu = a.slot + (size_t)w * 2 * ns;     // rows of the (spin-)empty sites
-    const int* sld = slu + ns;
+    const int* gku = a.kup + (size_t)w * ns;
+    const int* gkd = a.kdn + (size_t)w * ns;
+    const int* gsu = a.slot + (size_t)w * 2 * ns;     // rows of the (spin-)empty sites
+    const int* gsd = gsu + ns;
+    int16_t* sku = reinterpret_cast<int16_t*>(lds) + (size_t)wave * 4 * nsp;
+    int16_t* skd = sku + nsp;
+    int16_t* ssu = skd + nsp;
+    int16_t* ssd = ssu + nsp;
+    if (STAGE) {
+        for (int t = lane; t < ns; t += 64) {
+            sku[t] = (int16_t)gku[t];
+            skd[t] = (int16_t)gkd[t];
+            ssu[t] = (int16_t)gsu[t];
+            ssd[t] = (int16_t)gsd[t];
+        }
+        // the wavefront reads what its own lanes wrote
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    auto ku = [&](int x) -> int { return STAGE ? (int)sku[x] : gku[x]; };
+    auto kd = [&](int x) -> int { return STAGE ? (int)skd[x] : gkd[x]; };
+    auto su = [&](int x) -> int { return STAGE ? (int)ssu[x] : gsu[x]; };
+    auto sd = [&](int x) -> int { return STAGE ? (int)ssd[x] : gsd[x]; };
     const int c = a.npend[w];
     const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * a.delay * nsp;
     const T* gAd = gAu + (size_t)a.delay * nsp;
@@ -554,19 +577,20 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
     const T* gBd = gBu + (size_t)a.delay * nmax;
     double diag = 0.0, off = 0.0;
     unsigned long long nap = 0;
+#pragma unroll 2
     for (int b = lane; b < a.nb; b += 64) {
         const int2 bd = a.bonds[b];
         const int i = bd.x, j = bd.y;
-        const int iu = ku[i], ju = ku[j], id = kd[i], jd = kd[j];
+        const int iu = ku(i), ju = ku(j), id = kd(i), jd = kd(j);
         const bool t1 = ju != 0 && id != 0;   // S+_i S-_j
         const bool t2 = iu != 0 && jd != 0;   // S-_i S+_j
         // the target sites are empty for the moving spin (no site holds both)
         if (t1)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(slu[i], 0), ju - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sld[j], 0), id - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(su(i), 0), ju - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sd(j), 0), id - 1, c);
         if (t2)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(slu[j], 0), iu - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sld[i], 0), jd - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(su(j), 0), iu - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sd(i), 0), jd - 1, c);
         nap += (iu != 0 && jd != 0) || (ju != 0 && id != 0);
         const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
         diag += szi * szj;
@@ -584,8 +608,14 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
 
 hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s) {
     dim3 grid((a.nw + 3) / 4), block(256);
-    if (prec == 0) hipLaunchKernelGGL((k_eloc<double>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_eloc<float>), grid, block, 0, s, a);
+    const size_t lds = (size_t)4 * 4 * ((a.ns + 3) / 4 * 4) * sizeof(int16_t);   // 4 wavefronts x 4 arrays
+    if (lds <= 64 * 1024) {
+        if (prec == 0) hipLaunchKernelGGL((k_eloc<double, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_eloc<float, true>), grid, block, lds, s, a);
+    } else {
+        if (prec == 0) hipLaunchKernelGGL((k_eloc<double, false>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_eloc<float, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 
