@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 # rocprofv3 --pmc passes over this script's default FP32 run (summaries in
 # profiles/).
 REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_127_212}
-# the same for k_sweep, per walker-sweep (FP32, delay 8, L=12)
+# the same for k_sweep, per walker-sweep (FP32, delay 12, L=12)
 SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 21_291}
 
 # closed-shell boundary conditions per cluster size
@@ -374,7 +374,7 @@ def main():
     ap.add_argument("--L", type=int, default=12)
     ap.add_argument("--walkers", type=int, default=8192)
     ap.add_argument("--sweeps-per-step", type=int, default=0)
-    ap.add_argument("--delay", type=int, default=8)
+    ap.add_argument("--delay", type=int, default=12)
     ap.add_argument("--threads", type=int, default=0, help="sweep-kernel workgroup size (0: automatic)")
     ap.add_argument("--k", type=int, default=216, help="FP32 form: accepted moves per refresh")
     ap.add_argument("--k64", type=int, default=0,
@@ -447,7 +447,7 @@ def main():
 
     # the dominant kernel of the headline form carries the roofline
     rr = roofline_refresh(r32, ham.ns, REFRESH_TRAFFIC_BYTES_PER_WALKER.get(L))
-    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if args.delay == 8 else None)
+    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if args.delay == 12 else None)
     dominant = rr if rr and r32["kernel_ms"]["refresh_ms"] > r32["kernel_ms"]["sweep_ms"] else rs
     value = r32["walker_sweeps_per_s"]
     line = {

@@ -48,7 +48,7 @@ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // ---------------------------------------------------------------------------
 template <typename T, int V, int DMAX, int UNR = 4>
 __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __restrict__ sA,
-                                              const T* __restrict__ sB, int ld, int nsp,
+                                              const T* __restrict__ sB, int ld, int nsa,
                                               int nmax, int ncols, int c) {
     using VT = typename VecT<T, V>::type;   // UNR: columns in flight per thread
     const int nvec = ld / V;
@@ -60,7 +60,7 @@ __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __res
     VT a[DMAX];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q)
-        a[q] = (q < c) ? *reinterpret_cast<const VT*>(sA + (size_t)q * nsp + (size_t)iv * V) : VT(0);
+        a[q] = (q < c) ? *reinterpret_cast<const VT*>(sA + (size_t)q * nsa + (size_t)iv * V) : VT(0);
     T* __restrict__ base = W0 + (size_t)iv * V;
     int j = lane;
     for (; j + (UNR - 1) * lanes < ncols; j += UNR * lanes) {
@@ -91,9 +91,9 @@ __device__ __forceinline__ void stream_update(T* __restrict__ W0, const T* __res
 // W0[e, l] + sum_{q<c} A_q[e] B_q[l].
 template <typename T>
 __device__ __forceinline__ T w_cur(const T* __restrict__ W0, const T* sA, const T* sB, int ld,
-                                   int nsp, int nmax, int K, int l, int c) {
+                                   int nsa, int nmax, int K, int l, int c) {
     T v = W0[(size_t)l * ld + K];
-    for (int q = 0; q < c; ++q) v += sA[q * nsp + K] * sB[q * nmax + l];
+    for (int q = 0; q < c; ++q) v += sA[q * nsa + K] * sB[q * nmax + l];
     return v;
 }
 
@@ -107,7 +107,7 @@ __device__ __forceinline__ T w_cur(const T* __restrict__ W0, const T* sA, const 
 // correction or LDS store, so the accept costs one memory round trip.
 template <typename T>
 __device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T* __restrict__ Wdn,
-                                             T* sAu, T* sBu, T* sAd, T* sBd, int ldu, int ldd, int nsp,
+                                             T* sAu, T* sBu, T* sAd, T* sBd, int ldu, int ldd, int nsa,
                                              int nmax, int nup, int ndn, int Kup, int lup, int Kdn,
                                              int ldn, int c, T wu, T wd) {
     constexpr int CH = 4;
@@ -130,23 +130,23 @@ __device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T*
             const int t = base + t0 + u * bd;
             if (t < ldu) {
                 T vu = cu[u];
-                for (int q = 0; q < c; ++q) vu += sAu[q * nsp + t] * sBu[q * nmax + lup];
-                sAu[c * nsp + t] = vu + (t == Kup ? T(1) : T(0));
+                for (int q = 0; q < c; ++q) vu += sAu[q * nsa + t] * sBu[q * nmax + lup];
+                sAu[c * nsa + t] = vu + (t == Kup ? T(1) : T(0));
             }
             if (t < ldd) {
                 T vd = cd[u];
-                for (int q = 0; q < c; ++q) vd += sAd[q * nsp + t] * sBd[q * nmax + ldn];
-                sAd[c * nsp + t] = vd + (t == Kdn ? T(1) : T(0));
+                for (int q = 0; q < c; ++q) vd += sAd[q * nsa + t] * sBd[q * nmax + ldn];
+                sAd[c * nsa + t] = vd + (t == Kdn ? T(1) : T(0));
             }
             if (t < nup) {
                 T v = ru[u];
-                for (int q = 0; q < c; ++q) v += sAu[q * nsp + Kup] * sBu[q * nmax + t];
+                for (int q = 0; q < c; ++q) v += sAu[q * nsa + Kup] * sBu[q * nmax + t];
                 if (t == lup) v -= T(1);
                 sBu[c * nmax + t] = au * v;
             }
             if (t < ndn) {
                 T v = rd[u];
-                for (int q = 0; q < c; ++q) v += sAd[q * nsp + Kdn] * sBd[q * nmax + t];
+                for (int q = 0; q < c; ++q) v += sAd[q * nsa + Kdn] * sBd[q * nmax + t];
                 if (t == ldn) v -= T(1);
                 sBd[c * nmax + t] = ad * v;
             }
@@ -199,9 +199,9 @@ __device__ inline double block_sum_double(double v, double* red) {
 // same (uniform) control flow; proposals are evaluated redundantly so no
 // barrier is needed on rejected sweeps.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nmax, int delay, int nb) {
+__host__ __device__ inline size_t sweep_lds_base(size_t es, int ns, int nsa, int nmax, int delay, int nb) {
     const size_t nsp = (size_t)(ns + 3) / 4 * 4;
-    const size_t b = 4 * nsp * sizeof(int16_t) + 2 * (size_t)delay * (nsp + nmax) * es + 16 * sizeof(double) +
+    const size_t b = 4 * nsp * sizeof(int16_t) + 2 * (size_t)delay * ((size_t)nsa + nmax) * es + 16 * sizeof(double) +
                      (size_t)nb * sizeof(uint32_t) + (size_t)(ns + 1 + 2 * nb) * sizeof(uint16_t);
     return (b + 15) / 16 * 16;
 }
@@ -212,14 +212,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     const int w = a.walker0 + (int)blockIdx.x;
     const int ns = a.ns, nup = a.nup, ndn = a.ndn, nb = a.nb, nmax = a.nmax;
     const int nsp = round_up(ns, 4);
+    const int nsa = w_nsa(ns, nup, ndn);                 // stride of the pending columns
     const int delay = a.delay;
     // labels and rows fit 16 bits (ns <= 65535 is checked on the host)
     int16_t* s_kup = reinterpret_cast<int16_t*>(lds);
     int16_t* s_kdn = s_kup + nsp;
     int16_t* s_slu = s_kdn + nsp;                         // site -> row of W_up (-1: occupied)
     int16_t* s_sld = s_slu + nsp;
-    T* s_A = reinterpret_cast<T*>(s_sld + nsp);           // [2][delay][nsp] (8 nsp bytes: 16-byte aligned)
-    T* s_B = s_A + 2 * delay * nsp;                       // [2][delay][nmax]
+    T* s_A = reinterpret_cast<T*>(s_sld + nsp);           // [2][delay][nsa] (8 nsp bytes: 16-byte aligned)
+    T* s_B = s_A + 2 * delay * nsa;                       // [2][delay][nmax]
     double* s_red = reinterpret_cast<double*>(s_B + 2 * delay * nmax);
     uint32_t* s_bond = reinterpret_cast<uint32_t*>(s_red + 16);   // [nb] packed bond table
     uint16_t* s_noff = reinterpret_cast<uint16_t*>(s_bond + nb);    // [ns + 1]
@@ -229,13 +230,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     const int ldu = w_ld(ns, nup), ldd = w_ld(ns, ndn);
     T* const Wg = reinterpret_cast<T*>(a.W) + (size_t)w * wstride;
     // W of this walker: in HBM, or (RES) an LDS copy for the whole launch
-    T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nmax, delay, nb)) : Wg;
+    T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nsa, nmax, delay, nb)) : Wg;
     T* Wdn = Wup + w_off_dn(ns, nup);
     int* const slot_g = a.slot + (size_t)w * 2 * ns;
-    T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsp;
+    T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsa;
     T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
     T* sAu = s_A;
-    T* sAd = s_A + delay * nsp;
+    T* sAd = s_A + delay * nsa;
     T* sBu = s_B;
     T* sBd = s_B + delay * nmax;
 
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         s_sld[t] = (int16_t)slot_g[ns + t];
     }
     if (c > 0) {
-        for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
+        for (int t = (int)threadIdx.x; t < 2 * delay * nsa; t += (int)blockDim.x) s_A[t] = gA[t];
         for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
     }
     if (RES) {
@@ -328,8 +329,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
                 Kdn = can1 ? i : site;
                 // targets are empty for their spin; max() keeps a walker with an
                 // inconsistent state (failed refresh) inside W
-                wu = w_cur(Wup, sAu, sBu, ldu, nsp, nmax, max(s_slu[Kup], 0), lup, c);
-                wd = w_cur(Wdn, sAd, sBd, ldd, nsp, nmax, max(s_sld[Kdn], 0), ldn, c);
+                wu = w_cur(Wup, sAu, sBu, ldu, nsa, nmax, max(s_slu[Kup], 0), lup, c);
+                wd = w_cur(Wdn, sAd, sBd, ldd, nsa, nmax, max(s_sld[Kdn], 0), ldn, c);
                 dz = z_delta(s_bond, s_noff, s_nbr, s_kup, s_kdn, i, site);
                 const T ratio = wu * wd;
                 const double acc_prob = (double)ratio * (double)ratio;
@@ -365,7 +366,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
             ++n_acc;
             ++acc_since;
             const int eu = max(s_slu[tKup], 0), ed = max(s_sld[tKdn], 0);   // rows of the target sites
-            append_terms(Wup, Wdn, sAu, sBu, sAd, sBd, ldu, ldd, nsp, nmax, nup, ndn, eu, tlup, ed,
+            append_terms(Wup, Wdn, sAu, sBu, sAd, sBd, ldu, ldd, nsa, nmax, nup, ndn, eu, tlup, ed,
                          tldn, c, twu, twd);
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -383,8 +384,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
             }
             ++c;
             if (c == delay) {
-                stream_update<T, V, DMAX>(Wup, sAu, sBu, ldu, nsp, nmax, nup, c);
-                stream_update<T, V, DMAX>(Wdn, sAd, sBd, ldd, nsp, nmax, ndn, c);
+                stream_update<T, V, DMAX>(Wup, sAu, sBu, ldu, nsa, nmax, nup, c);
+                stream_update<T, V, DMAX>(Wdn, sAd, sBd, ldd, nsa, nmax, ndn, c);
                 c = 0;
                 ++n_flush;
             }
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wg[t] = Wup[t];
     }
     if (c > 0) {
-        for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) gA[t] = s_A[t];
+        for (int t = (int)threadIdx.x; t < 2 * delay * nsa; t += (int)blockDim.x) gA[t] = s_A[t];
         for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) gB[t] = s_B[t];
     }
     if (threadIdx.x == 0) {
@@ -439,7 +440,7 @@ bool sweep_resident(int prec, int ns, int nup, int ndn) {
 size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay, int nb) {
     const int nmax = nup > ndn ? nup : ndn;
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
-    return sweep_lds_base(es, ns, nmax, delay, nb) +
+    return sweep_lds_base(es, ns, w_nsa(ns, nup, ndn), nmax, delay, nb) +
            (sweep_resident(prec, ns, nup, ndn) ? w_stride(ns, nup, ndn) * es : 0);
 }
 
@@ -464,6 +465,7 @@ static hipError_t launch_sweep_v(const SweepArgs& a, int nw, int threads, size_t
     if (a.delay <= 1) return launch_sweep_d<T, V, 1>(a, nw, threads, lds, s);
     if (a.delay <= 4) return launch_sweep_d<T, V, 4>(a, nw, threads, lds, s);
     if (a.delay <= 8) return launch_sweep_d<T, V, 8>(a, nw, threads, lds, s);
+    if (a.delay <= 12) return launch_sweep_d<T, V, 12>(a, nw, threads, lds, s);
     return launch_sweep_d<T, V, kMaxDelay>(a, nw, threads, lds, s);
 }
 
@@ -483,27 +485,27 @@ __global__ void __launch_bounds__(1024) k_flush(SweepArgs a) {
     const int w = a.walker0 + (int)blockIdx.x;
     const int c = a.npend[w];
     if (c == 0) return;
-    const int ns = a.ns, nsp = round_up(ns, 4), nmax = a.nmax, delay = a.delay;
+    const int ns = a.ns, nsa = w_nsa(ns, a.nup, a.ndn), nmax = a.nmax, delay = a.delay;
     T* s_A = reinterpret_cast<T*>(lds);
-    T* s_B = s_A + 2 * delay * nsp;
-    const T* gA = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * delay * nsp;
+    T* s_B = s_A + 2 * delay * nsa;
+    const T* gA = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * delay * nsa;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
-    for (int t = (int)threadIdx.x; t < 2 * delay * nsp; t += (int)blockDim.x) s_A[t] = gA[t];
+    for (int t = (int)threadIdx.x; t < 2 * delay * nsa; t += (int)blockDim.x) s_A[t] = gA[t];
     for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
     __syncthreads();
     T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
     T* Wdn = Wup + w_off_dn(ns, a.nup);
-    stream_update<T, V, kMaxDelay, 2>(Wup, s_A, s_B, w_ld(ns, a.nup), nsp, nmax, a.nup, c);
-    stream_update<T, V, kMaxDelay, 2>(Wdn, s_A + delay * nsp, s_B + delay * nmax, w_ld(ns, a.ndn), nsp,
+    stream_update<T, V, kMaxDelay, 2>(Wup, s_A, s_B, w_ld(ns, a.nup), nsa, nmax, a.nup, c);
+    stream_update<T, V, kMaxDelay, 2>(Wdn, s_A + delay * nsa, s_B + delay * nmax, w_ld(ns, a.ndn), nsa,
                                       nmax, a.ndn, c);
     __syncthreads();
     if (threadIdx.x == 0) a.npend[w] = 0;
 }
 
 hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
-    const int nsp = (a.ns + 3) / 4 * 4;
+    const int nsa = w_nsa(a.ns, a.nup, a.ndn);
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
-    const size_t lds = (size_t)2 * a.delay * (nsp + a.nmax) * es;
+    const size_t lds = (size_t)2 * a.delay * (nsa + a.nmax) * es;
     const int V = prec == 0 ? 2 : 4;
     int threads = 256;
     const int ldm = w_ld(a.ns, a.nup) > w_ld(a.ns, a.ndn) ? w_ld(a.ns, a.nup) : w_ld(a.ns, a.ndn);
@@ -527,10 +529,10 @@ hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
 // stride over the bond list, FP64 accumulation, shuffle reduction.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ld, int nsp,
+__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ld, int nsa,
                                          int nmax, int K, int l, int c) {
     double v = (double)W0[(size_t)l * ld + K];
-    for (int q = 0; q < c; ++q) v += (double)(gA[q * nsp + K] * gB[q * nmax + l]);
+    for (int q = 0; q < c; ++q) v += (double)(gA[q * nsa + K] * gB[q * nmax + l]);
     return v;
 }
 
@@ -571,8 +573,9 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
     auto su = [&](int x) -> int { return STAGE ? (int)ssu[x] : gsu[x]; };
     auto sd = [&](int x) -> int { return STAGE ? (int)ssd[x] : gsd[x]; };
     const int c = a.npend[w];
-    const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * a.delay * nsp;
-    const T* gAd = gAu + (size_t)a.delay * nsp;
+    const int nsa = w_nsa(ns, a.nup, a.ndn);
+    const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * a.delay * nsa;
+    const T* gAd = gAu + (size_t)a.delay * nsa;
     const T* gBu = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * a.delay * nmax;
     const T* gBd = gBu + (size_t)a.delay * nmax;
     double diag = 0.0, off = 0.0;
@@ -586,11 +589,11 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
         const bool t2 = iu != 0 && jd != 0;   // S-_i S+_j
         // the target sites are empty for the moving spin (no site holds both)
         if (t1)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(su(i), 0), ju - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sd(j), 0), id - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsa, nmax, max(su(i), 0), ju - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, nsa, nmax, max(sd(j), 0), id - 1, c);
         if (t2)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsp, nmax, max(su(j), 0), iu - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ldd, nsp, nmax, max(sd(i), 0), jd - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsa, nmax, max(su(j), 0), iu - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, nsa, nmax, max(sd(i), 0), jd - 1, c);
         nap += (iu != 0 && jd != 0) || (ju != 0 && id != 0);
         const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
         diag += szi * szj;

@@ -29,6 +29,11 @@ constexpr int kMaxDelay = 16;
 // site the electron left.
 __host__ __device__ inline int w_ld(int ns, int n) { return (ns - n + 3) / 4 * 4; }
 __host__ __device__ inline size_t w_off_dn(int ns, int nup) { return (size_t)w_ld(ns, nup) * nup; }
+// Stride of the pending columns A_q (one entry per row of the taller W).
+__host__ __device__ inline int w_nsa(int ns, int nup, int ndn) {
+    const int a = w_ld(ns, nup), b = w_ld(ns, ndn);
+    return (a > b ? a : b) > 4 ? (a > b ? a : b) : 4;
+}
 __host__ __device__ inline size_t w_stride(int ns, int nup, int ndn) {
     return w_off_dn(ns, nup) + (size_t)w_ld(ns, ndn) * ndn;
 }
@@ -71,7 +76,7 @@ struct SweepArgs {
     int* kdn;
     void* W;                        // [nw][w_stride] empty-site rows, column-major per spin
     int* slot;                      // [nw][2][ns] site -> row of W (-1: occupied)
-    void* A;                        // [nw][2][delay][nsp]  pending columns (over rows of W)
+    void* A;                        // [nw][2][delay][w_nsa]  pending columns (over rows of W)
     void* Bm;                       // [nw][2][delay][nmax] pending scaled rows
     int* npend;                     // [nw] pending count
     int delay;                      // 1 = immediate rank-1 update

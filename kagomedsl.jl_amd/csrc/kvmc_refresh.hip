@@ -607,9 +607,9 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     constexpr int SB = 16;
     const int xs = (nc16 | 1);                       // LDS row stride (odd)
     double* xb = stage;                              // [SB][xs]
-    const int nsp = rup_i(ns, 4);
+    const int nsa = w_nsa(ns, a.nup, a.ndn);
     const int cpend = a.write_w ? 0 : a.npend[w];
-    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsa;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     for (int sb = 0; sb < ns; sb += SB) {
@@ -631,7 +631,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 *dst = (T)v;
             } else {
                 double cur = (double)*dst;
-                for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsp + row] * (double)gB[c * nmax + l];
+                for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsa + row] * (double)gB[c * nmax + l];
                 dmax = fmax(dmax, fabs(v - cur));
                 vmax = fmax(vmax, fabs(v));
             }
@@ -852,9 +852,9 @@ k_refresh_solve(RefreshArgs a, int groups) {
                          (spin ? w_off_dn(ns, a.nup) : 0);
     const int ld = w_ld(ns, N);
     int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * ns;
-    const int nsp = rup_i(ns, 4);
+    const int nsa = w_nsa(ns, a.nup, a.ndn);
     const int cpend = a.write_w ? 0 : a.npend[w];
-    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsp;
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsa;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     if (site >= 0) {
@@ -872,7 +872,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
                         *dst = (T)v;
                     } else {
                         double cur = (double)*dst;
-                        for (int q = 0; q < cpend; ++q) cur += (double)gA[q * nsp + row] * (double)gB[q * nmax + l];
+                        for (int q = 0; q < cpend; ++q) cur += (double)gA[q * nsa + row] * (double)gB[q * nmax + l];
                         dmax = fmax(dmax, fabs(v - cur));
                         vmax = fmax(vmax, fabs(v));
                     }

@@ -30,7 +30,7 @@ def candidates(ns: int, precision: int):
     nvec = ns // vec if ns % vec == 0 else ns
     floor = (nvec + 63) // 64 * 64
     threads = sorted({max(t, floor) for t in (64, 128, 256) if max(t, floor) <= 512})
-    delays = (4, 6, 8) if precision == nat.PREC_FP32 else (1,)
+    delays = (4, 8, 12) if precision == nat.PREC_FP32 else (1,)
     return [(t, d) for t in threads for d in delays]
 
 

@@ -216,7 +216,7 @@ def test_philox_chain_parity(oracle_mod):
         np.testing.assert_array_equal(kfu[w], ch.kup)
 
 
-@pytest.mark.parametrize("delay", [1, 4, 16])
+@pytest.mark.parametrize("delay", [1, 4, 8, 12, 16])
 def test_fp32_delayed_update_tracks_fp64(oracle_mod, delay):
     """FP32 W with `delay` pending rank-1 terms: W after the same tape agrees
     with the FP64 oracle chain to FP32 accuracy, decisions coincide except
