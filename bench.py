@@ -44,7 +44,7 @@ FP64_PEAK_TFLOPS = 78.6
 # profiles/).
 REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_127_212}
 # the same for k_sweep, per walker-sweep (FP32, delay 12, L=12)
-SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 21_291}
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 17_997}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),

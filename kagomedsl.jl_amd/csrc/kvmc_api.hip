@@ -369,8 +369,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         ALLOC(p, wbytes); h->W = p;
         // zero once: the padding rows of W must stay zero (no pending term touches them)
         if (hipMemset(p, 0, wbytes) != hipSuccess) { free_all(h); delete h; return fail(KVMC_ERR_HIP, "hipMemset of W failed"); }
-        ALLOC(p, nw * 2 * delay * (size_t)w_nsa(h->ns, h->nup, h->ndn) * h->esize); h->A = p;
-        ALLOC(p, nw * 2 * delay * h->nmax * h->esize); h->Bm = p;
+        ALLOC(p, nw * 2 * pend_ld(delay) * (size_t)w_nsa(h->ns, h->nup, h->ndn) * h->esize); h->A = p;
+        ALLOC(p, nw * 2 * pend_ld(delay) * h->nmax * h->esize); h->Bm = p;
     }
     ALLOC(h->npend, nw); ALLOC(h->zmu, nw); ALLOC(h->acc_since, nw); ALLOC(h->status, nw);
     ALLOC(h->sweeps, nw); ALLOC(h->stop, nw); ALLOC(h->tbase, nw);

@@ -154,6 +154,43 @@ __device__ __forceinline__ void append_terms(const T* __restrict__ Wup, const T*
     }
 }
 
+// Pending terms between HBM ([spin][row][dp], one row's terms contiguous)
+// and LDS ([spin][q][row], one term's column contiguous); only the c live
+// terms move.  One thread per row of A or label of B.
+template <typename T>
+__device__ __forceinline__ void pend_load(T* s_A, T* s_B, const T* gA, const T* gB, int nsa,
+                                          int nmax, int delay, int dp, int c) {
+    for (int t = (int)threadIdx.x; t < 2 * nsa; t += (int)blockDim.x) {
+        const int sp = t >= nsa ? 1 : 0;
+        T* d = s_A + sp * delay * nsa + (t - sp * nsa);
+        const T* g = gA + (size_t)t * dp;
+        for (int q = 0; q < c; ++q) d[q * nsa] = g[q];
+    }
+    for (int t = (int)threadIdx.x; t < 2 * nmax; t += (int)blockDim.x) {
+        const int sp = t >= nmax ? 1 : 0;
+        T* d = s_B + sp * delay * nmax + (t - sp * nmax);
+        const T* g = gB + (size_t)t * dp;
+        for (int q = 0; q < c; ++q) d[q * nmax] = g[q];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void pend_store(const T* s_A, const T* s_B, T* gA, T* gB, int nsa,
+                                           int nmax, int delay, int dp, int c) {
+    for (int t = (int)threadIdx.x; t < 2 * nsa; t += (int)blockDim.x) {
+        const int sp = t >= nsa ? 1 : 0;
+        const T* d = s_A + sp * delay * nsa + (t - sp * nsa);
+        T* g = gA + (size_t)t * dp;
+        for (int q = 0; q < c; ++q) g[q] = d[q * nsa];
+    }
+    for (int t = (int)threadIdx.x; t < 2 * nmax; t += (int)blockDim.x) {
+        const int sp = t >= nmax ? 1 : 0;
+        const T* d = s_B + sp * delay * nmax + (t - sp * nmax);
+        T* g = gB + (size_t)t * dp;
+        for (int q = 0; q < c; ++q) g[q] = d[q * nmax];
+    }
+}
+
 // Change of Z (src/MonteCarlo.jl:460-474) when the electrons on the
 // antiparallel pair (i, j) are exchanged: at half filling every site holds
 // one electron, so each bond touching exactly one of i and j flips its
@@ -233,8 +270,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     T* Wup = RES ? reinterpret_cast<T*>(lds + sweep_lds_base(sizeof(T), ns, nsa, nmax, delay, nb)) : Wg;
     T* Wdn = Wup + w_off_dn(ns, nup);
     int* const slot_g = a.slot + (size_t)w * 2 * ns;
-    T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * delay * nsa;
-    T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
+    const int dp = pend_ld(delay);
+    T* gA = reinterpret_cast<T*>(a.A) + (size_t)w * 2 * nsa * dp;
+    T* gB = reinterpret_cast<T*>(a.Bm) + (size_t)w * 2 * nmax * dp;
     T* sAu = s_A;
     T* sAd = s_A + delay * nsa;
     T* sBu = s_B;
@@ -252,10 +290,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         s_slu[t] = (int16_t)slot_g[t];
         s_sld[t] = (int16_t)slot_g[ns + t];
     }
-    if (c > 0) {
-        for (int t = (int)threadIdx.x; t < 2 * delay * nsa; t += (int)blockDim.x) s_A[t] = gA[t];
-        for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
-    }
+    if (c > 0) pend_load(s_A, s_B, gA, gB, nsa, nmax, delay, dp, c);
     if (RES) {
         for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wup[t] = Wg[t];
     }
@@ -409,10 +444,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
     if (RES) {
         for (size_t t = threadIdx.x; t < wstride; t += blockDim.x) Wg[t] = Wup[t];
     }
-    if (c > 0) {
-        for (int t = (int)threadIdx.x; t < 2 * delay * nsa; t += (int)blockDim.x) gA[t] = s_A[t];
-        for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) gB[t] = s_B[t];
-    }
+    if (c > 0) pend_store(s_A, s_B, gA, gB, nsa, nmax, delay, dp, c);
     if (threadIdx.x == 0) {
         a.sweeps[w] = sweeps;
         a.npend[w] = c;
@@ -488,10 +520,10 @@ __global__ void __launch_bounds__(1024) k_flush(SweepArgs a) {
     const int ns = a.ns, nsa = w_nsa(ns, a.nup, a.ndn), nmax = a.nmax, delay = a.delay;
     T* s_A = reinterpret_cast<T*>(lds);
     T* s_B = s_A + 2 * delay * nsa;
-    const T* gA = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * delay * nsa;
-    const T* gB = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * delay * nmax;
-    for (int t = (int)threadIdx.x; t < 2 * delay * nsa; t += (int)blockDim.x) s_A[t] = gA[t];
-    for (int t = (int)threadIdx.x; t < 2 * delay * nmax; t += (int)blockDim.x) s_B[t] = gB[t];
+    const int dp = pend_ld(delay);
+    const T* gA = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * nsa * dp;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * nmax * dp;
+    pend_load(s_A, s_B, gA, gB, nsa, nmax, delay, dp, c);
     __syncthreads();
     T* Wup = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
     T* Wdn = Wup + w_off_dn(ns, a.nup);
@@ -528,11 +560,23 @@ hipError_t launch_flush(int prec, const SweepArgs& a, int nw, hipStream_t s) {
 // :636-672 and SzInteraction! :593-604): one wavefront per walker, lanes
 // stride over the bond list, FP64 accumulation, shuffle reduction.
 // ---------------------------------------------------------------------------
+// The c pending terms of row K and label l are contiguous (pend_ld apart
+// per row), read as 16-byte vectors; the slots past c are never summed.
 template <typename T>
-__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ld, int nsa,
-                                         int nmax, int K, int l, int c) {
+__device__ __forceinline__ double w_elem(const T* W0, const T* gA, const T* gB, int ld, int dp,
+                                         int K, int l, int c) {
+    using VT = typename VecT<T, native_vec<T>()>::type;
+    constexpr int NV = native_vec<T>();
     double v = (double)W0[(size_t)l * ld + K];
-    for (int q = 0; q < c; ++q) v += (double)(gA[q * nsa + K] * gB[q * nmax + l]);
+    const T* ra = gA + (size_t)K * dp;
+    const T* rb = gB + (size_t)l * dp;
+    for (int q0 = 0; q0 < c; q0 += NV) {
+        const VT x = *reinterpret_cast<const VT*>(ra + q0);
+        const VT y = *reinterpret_cast<const VT*>(rb + q0);
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+            if (q0 + u < c) v += (double)(x[u] * y[u]);
+    }
     return v;
 }
 
@@ -574,10 +618,11 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
     auto sd = [&](int x) -> int { return STAGE ? (int)ssd[x] : gsd[x]; };
     const int c = a.npend[w];
     const int nsa = w_nsa(ns, a.nup, a.ndn);
-    const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * a.delay * nsa;
-    const T* gAd = gAu + (size_t)a.delay * nsa;
-    const T* gBu = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * a.delay * nmax;
-    const T* gBd = gBu + (size_t)a.delay * nmax;
+    const int dp = pend_ld(a.delay);
+    const T* gAu = reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * nsa * dp;
+    const T* gAd = gAu + (size_t)nsa * dp;
+    const T* gBu = reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * nmax * dp;
+    const T* gBd = gBu + (size_t)nmax * dp;
     double diag = 0.0, off = 0.0;
     unsigned long long nap = 0;
 #pragma unroll 2
@@ -589,11 +634,11 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
         const bool t2 = iu != 0 && jd != 0;   // S-_i S+_j
         // the target sites are empty for the moving spin (no site holds both)
         if (t1)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsa, nmax, max(su(i), 0), ju - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ldd, nsa, nmax, max(sd(j), 0), id - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, dp, max(su(i), 0), ju - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, dp, max(sd(j), 0), id - 1, c);
         if (t2)
-            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, nsa, nmax, max(su(j), 0), iu - 1, c) *
-                   w_elem(Wdn, gAd, gBd, ldd, nsa, nmax, max(sd(i), 0), jd - 1, c);
+            off += -0.5 * w_elem(Wup, gAu, gBu, ldu, dp, max(su(j), 0), iu - 1, c) *
+                   w_elem(Wdn, gAd, gBd, ldd, dp, max(sd(i), 0), jd - 1, c);
         nap += (iu != 0 && jd != 0) || (ju != 0 && id != 0);
         const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
         diag += szi * szj;

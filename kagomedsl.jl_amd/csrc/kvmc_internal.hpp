@@ -29,7 +29,11 @@ constexpr int kMaxDelay = 16;
 // site the electron left.
 __host__ __device__ inline int w_ld(int ns, int n) { return (ns - n + 3) / 4 * 4; }
 __host__ __device__ inline size_t w_off_dn(int ns, int nup) { return (size_t)w_ld(ns, nup) * nup; }
-// Stride of the pending columns A_q (one entry per row of the taller W).
+// Pending rank-1 terms in HBM: A as [walker][spin][row < w_nsa][pend_ld],
+// B as [walker][spin][label < nmax][pend_ld], so the terms of one row (or
+// label) are contiguous and 16-byte aligned.
+__host__ __device__ inline int pend_ld(int delay) { return (delay + 3) / 4 * 4; }
+// Number of rows of the pending columns A_q (the rows of the taller W).
 __host__ __device__ inline int w_nsa(int ns, int nup, int ndn) {
     const int a = w_ld(ns, nup), b = w_ld(ns, ndn);
     return (a > b ? a : b) > 4 ? (a > b ? a : b) : 4;
@@ -76,8 +80,8 @@ struct SweepArgs {
     int* kdn;
     void* W;                        // [nw][w_stride] empty-site rows, column-major per spin
     int* slot;                      // [nw][2][ns] site -> row of W (-1: occupied)
-    void* A;                        // [nw][2][delay][w_nsa]  pending columns (over rows of W)
-    void* Bm;                       // [nw][2][delay][nmax] pending scaled rows
+    void* A;                        // [nw][2][w_nsa][pend_ld]  pending columns (over rows of W)
+    void* Bm;                       // [nw][2][nmax][pend_ld] pending scaled rows
     int* npend;                     // [nw] pending count
     int delay;                      // 1 = immediate rank-1 update
     int nmax;                       // max(nup, ndn)

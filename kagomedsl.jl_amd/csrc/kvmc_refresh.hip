@@ -609,8 +609,9 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     double* xb = stage;                              // [SB][xs]
     const int nsa = w_nsa(ns, a.nup, a.ndn);
     const int cpend = a.write_w ? 0 : a.npend[w];
-    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsa;
-    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
+    const int dp = pend_ld(a.delay);
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * nmax * dp;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     for (int sb = 0; sb < ns; sb += SB) {
         for (int i = wave; i < SB; i += NWAVE) {
@@ -631,7 +632,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 *dst = (T)v;
             } else {
                 double cur = (double)*dst;
-                for (int c = 0; c < cpend; ++c) cur += (double)gA[c * nsa + row] * (double)gB[c * nmax + l];
+                for (int c = 0; c < cpend; ++c) cur += (double)gA[(size_t)row * dp + c] * (double)gB[(size_t)l * dp + c];
                 dmax = fmax(dmax, fabs(v - cur));
                 vmax = fmax(vmax, fabs(v));
             }
@@ -854,8 +855,9 @@ k_refresh_solve(RefreshArgs a, int groups) {
     int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * ns;
     const int nsa = w_nsa(ns, a.nup, a.ndn);
     const int cpend = a.write_w ? 0 : a.npend[w];
-    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * a.delay * nsa;
-    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.delay * nmax;
+    const int dp = pend_ld(a.delay);
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * nmax * dp;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     if (site >= 0) {
         const int row = a.write_w ? e : max(sl[site], 0);
@@ -872,7 +874,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
                         *dst = (T)v;
                     } else {
                         double cur = (double)*dst;
-                        for (int q = 0; q < cpend; ++q) cur += (double)gA[q * nsa + row] * (double)gB[q * nmax + l];
+                        for (int q = 0; q < cpend; ++q) cur += (double)gA[(size_t)row * dp + q] * (double)gB[(size_t)l * dp + q];
                         dmax = fmax(dmax, fabs(v - cur));
                         vmax = fmax(vmax, fabs(v));
                     }
