@@ -42,9 +42,9 @@ FP64_PEAK_TFLOPS = 78.6
 # walker refreshed by the kernel pair k_refresh_blocked + k_refresh_solve, from
 # rocprofv3 --pmc passes over this script's default FP32 run (summaries in
 # profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_127_212}
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_116_949}
 # the same for k_sweep, per walker-sweep (FP32, delay 12, L=12)
-SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 17_997}
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 18_014}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),

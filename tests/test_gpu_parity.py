@@ -231,6 +231,10 @@ def test_fp32_delayed_update_tracks_fp64(oracle_mod, delay):
         e32, _ = ds.measure()
         kfu, kfd = ds.get_configs()
         W32 = [ds.get_w(w, 0) for w in range(nw)]
+        # get_w flushed every walker: E_loc from the flushed W equals E_loc
+        # with the pending terms folded in, to FP32 rounding
+        e32_flushed, _ = ds.measure()
+    np.testing.assert_allclose(e32_flushed, e32, rtol=2e-5, atol=2e-5)
     same = 0
     for w in range(nw):
         ch = oracle_mod.Chain.start(ham.nn, ham.U_up, ham.U_down, ku[w], kd[w])
