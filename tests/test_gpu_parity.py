@@ -249,6 +249,50 @@ def test_fp32_delayed_update_tracks_fp64(oracle_mod, delay):
         assert cnt["flushes"] <= cnt["accepted"] // delay + nw
 
 
+@pytest.mark.parametrize("n_up", [20, 30])
+def test_fp32_delayed_update_unequal_filling(oracle_mod, n_up):
+    """FP32 W at delay 12 with N_up != N_down at L=4: the two spins' W have
+    different row counts (28 and 20 empty-site rows for n_up = 20), and the
+    pending columns of both use the taller one's stride.  W and E_L track
+    the FP64 oracle chain on the same tape, and E_L with the pending terms
+    folded in equals E_L after a flush."""
+    from kagome_vmc import DoubleKagome, Hamiltonian
+    lat = DoubleKagome.build(1.0, 4, 4, (True, True), (False, True))
+    ham = Hamiltonian.build(n_up, 48 - n_up, lat)
+    nw, nsweeps = 6, 300
+    ku = np.zeros((nw, 48), dtype=np.int32)
+    kd = np.zeros((nw, 48), dtype=np.int32)
+    w = q = 0
+    while w < nw:
+        a, b = oracle_mod.random_config(5, q, 0, 48, n_up, 48 - n_up)
+        q += 1
+        if max(np.linalg.cond(oracle_mod.tilde_u(ham.U_up, a)),
+               np.linalg.cond(oracle_mod.tilde_u(ham.U_down, b))) < 1e5:
+            ku[w], kd[w] = a, b
+            w += 1
+    tu, tb = _tape(nw, nsweeps, len(ham.nn), 23)
+    with sampler(ham, nw, precision=1, delay=12, refresh_policy=1, refresh_k=10**6) as ds:
+        ds.set_configs(ku, kd)
+        cnt, out = ds.sweep(nsweeps, tu, tb, record=True)
+        e32, _ = ds.measure()
+        Wu = [ds.get_w(w, 0) for w in range(nw)]
+        Wd = [ds.get_w(w, 1) for w in range(nw)]
+        e32_flushed, _ = ds.measure()
+    np.testing.assert_allclose(e32_flushed, e32, rtol=2e-5, atol=2e-5)
+    assert cnt["accepted"] > 12 * nw // 2   # several flushes and live pending terms
+    same = 0
+    for w in range(nw):
+        ch = oracle_mod.Chain.start(ham.nn, ham.U_up, ham.U_down, ku[w], kd[w])
+        oc, _, _ = oracle_mod.run_chain(ch, nsweeps, tape_u=tu[w], tape_bond=tb[w], policy=1, k=10**6)
+        if np.array_equal(out[w], oc):
+            same += 1
+            assert _max_rel(Wu[w], ch.Wup.real) < 2e-4
+            assert _max_rel(Wd[w], ch.Wdn.real) < 2e-4
+            e_or = oracle_mod.getol(ham.nn, ch.kup, ch.kdn, ch.Wup, ch.Wdn)
+            assert abs(e32[w] - e_or) < 1e-4 * max(1.0, abs(e_or))
+    assert same >= nw - 1
+
+
 def test_refresh_cadence_reference_policy(oracle_mod):
     """Reference cadence: a refresh after every sweep with counter % N == 0
     that passed both early returns (src/MonteCarlo.jl:593-604)."""
