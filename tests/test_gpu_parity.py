@@ -42,6 +42,33 @@ def test_refresh_matches_oracle(oracle_mod, L, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_refresh_many_walkers(oracle_mod, prec):
+    """520 walkers refreshed in one launch pair: every walker's W and
+    log-determinant equal the oracle's after set_configs, and W matches the
+    oracle's refresh of the final configuration after sweeps that refresh
+    most walkers at the end of the call (refresh_k = 1)."""
+    ham = system(4)
+    nw = 520
+    ku, kd = well_conditioned_configs(oracle_mod, 13, nw, ham, max_cond=1e5)
+    tol = W_RTOL if prec == 0 else 1e-6
+    with sampler(ham, nw, precision=prec, refresh_policy=2, refresh_k=1, delay=1 if prec == 0 else 4) as ds:
+        ds.set_configs(ku, kd)
+        logdet = ds.logdet()
+        for w in range(nw):
+            Wu_o, lu = oracle_mod.refresh_w(ham.U_up, ku[w], with_logdet=True)
+            Wd_o, ld = oracle_mod.refresh_w(ham.U_down, kd[w], with_logdet=True)
+            assert _max_rel(ds.get_w(w, 0), Wu_o.real) < tol
+            assert _max_rel(ds.get_w(w, 1), Wd_o.real) < tol
+            assert abs(logdet[w] - (lu + ld)) < 1e-9 * max(1.0, abs(lu + ld))
+        cnt, _ = ds.sweep(20)
+        assert cnt["refreshes"] >= 256
+        ku2, kd2 = ds.get_configs()
+        for w in range(0, nw, 7):
+            assert _max_rel(ds.get_w(w, 0), oracle_mod.refresh_w(ham.U_up, ku2[w]).real) < 100 * tol
+            assert _max_rel(ds.get_w(w, 1), oracle_mod.refresh_w(ham.U_down, kd2[w]).real) < 100 * tol
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_refresh_above_256_electrons(oracle_mod, prec):
     """N_up = 264 at L=12 (432 sites, N_down = 168): beyond 256 electrons of
     one spin the refresh runs as a single kernel over [tildeU; U_empty].
