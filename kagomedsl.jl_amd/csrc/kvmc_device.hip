@@ -654,7 +654,80 @@ __global__ void __launch_bounds__(256) k_eloc(ElocArgs a) {
     }
 }
 
+// FP32 with delayed updates: one workgroup per walker.  The labels, row
+// maps and the walker's pending terms (A as [spin][row][dp], B as
+// [spin][label][dp], both contiguous in HBM) are copied to LDS once, with
+// 16-byte loads, so the bond loop gathers only W from memory; the 256 lanes
+// split the bonds and a block reduction forms E_L.
+static size_t eloc_wg_lds(const ElocArgs& a) {
+    const size_t nsp = (size_t)(a.ns + 3) / 4 * 4, dp = (size_t)pend_ld(a.delay);
+    return 8 * nsp + (2 * (size_t)w_nsa(a.ns, a.nup, a.ndn) + 2 * (size_t)a.nmax) * dp * sizeof(float) +
+           16 * sizeof(double);
+}
+
+__global__ void __launch_bounds__(256) k_eloc_wg(ElocArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    using T = float;
+    const int w = (int)blockIdx.x;
+    const int ns = a.ns, nmax = a.nmax, nsp = round_up(ns, 4);
+    const int nsa = w_nsa(ns, a.nup, a.ndn), dp = pend_ld(a.delay);
+    const int ldu = w_ld(ns, a.nup), ldd = w_ld(ns, a.ndn);
+    const T* Wup = reinterpret_cast<const T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
+    const T* Wdn = Wup + w_off_dn(ns, a.nup);
+    int16_t* sku = reinterpret_cast<int16_t*>(lds);
+    int16_t* skd = sku + nsp;
+    int16_t* ssu = skd + nsp;
+    int16_t* ssd = ssu + nsp;
+    T* sAu = reinterpret_cast<T*>(ssd + nsp);             // [2][nsa][dp] (8 nsp bytes: 16-byte aligned)
+    T* sAd = sAu + (size_t)nsa * dp;
+    T* sBu = sAu + (size_t)2 * nsa * dp;                  // [2][nmax][dp]
+    T* sBd = sBu + (size_t)nmax * dp;
+    double* red = reinterpret_cast<double*>(sBu + (size_t)2 * nmax * dp);
+    const int t0 = (int)threadIdx.x, bd = (int)blockDim.x;
+    for (int t = t0; t < ns; t += bd) {
+        sku[t] = (int16_t)a.kup[(size_t)w * ns + t];
+        skd[t] = (int16_t)a.kdn[(size_t)w * ns + t];
+        ssu[t] = (int16_t)a.slot[(size_t)w * 2 * ns + t];
+        ssd[t] = (int16_t)a.slot[(size_t)w * 2 * ns + ns + t];
+    }
+    const int c = a.npend[w];
+    if (c > 0) {
+        const f32x4* gA = reinterpret_cast<const f32x4*>(reinterpret_cast<const T*>(a.A) + (size_t)w * 2 * nsa * dp);
+        const f32x4* gB = reinterpret_cast<const f32x4*>(reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * nmax * dp);
+        f32x4* dA = reinterpret_cast<f32x4*>(sAu);
+        f32x4* dB = reinterpret_cast<f32x4*>(sBu);
+        for (int t = t0; t < 2 * nsa * dp / 4; t += bd) dA[t] = gA[t];
+        for (int t = t0; t < 2 * nmax * dp / 4; t += bd) dB[t] = gB[t];
+    }
+    __syncthreads();
+    double diag = 0.0, off = 0.0, nap = 0.0;
+    for (int b = t0; b < a.nb; b += bd) {
+        const int2 bb = a.bonds[b];
+        const int i = bb.x, j = bb.y;
+        const int iu = sku[i], ju = sku[j], id = skd[i], jd = skd[j];
+        if (ju != 0 && id != 0)   // S+_i S-_j; the target sites are empty for the moving spin
+            off += -0.5 * w_elem(Wup, sAu, sBu, ldu, dp, max((int)ssu[i], 0), ju - 1, c) *
+                   w_elem(Wdn, sAd, sBd, ldd, dp, max((int)ssd[j], 0), id - 1, c);
+        if (iu != 0 && jd != 0)   // S-_i S+_j
+            off += -0.5 * w_elem(Wup, sAu, sBu, ldu, dp, max((int)ssu[j], 0), iu - 1, c) *
+                   w_elem(Wdn, sAd, sBd, ldd, dp, max((int)ssd[i], 0), jd - 1, c);
+        nap += ((iu != 0 && jd != 0) || (ju != 0 && id != 0)) ? 1.0 : 0.0;
+        const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
+        diag += szi * szj;
+    }
+    const double v = block_sum_double(diag + off, red);
+    const double napt = block_sum_double(nap, red + 8);
+    if (t0 == 0) {
+        a.eloc[w] = v;
+        atomicAdd(&a.counters[CNT_NAP], (unsigned long long)napt);
+    }
+}
+
 hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s) {
+    if (prec == 1 && a.delay > 1 && eloc_wg_lds(a) <= 64 * 1024) {
+        hipLaunchKernelGGL(k_eloc_wg, dim3(a.nw), dim3(256), eloc_wg_lds(a), s, a);
+        return hipGetLastError();
+    }
     dim3 grid((a.nw + 3) / 4), block(256);
     const size_t lds = (size_t)4 * 4 * ((a.ns + 3) / 4 * 4) * sizeof(int16_t);   // 4 wavefronts x 4 arrays
     if (lds <= 64 * 1024) {
