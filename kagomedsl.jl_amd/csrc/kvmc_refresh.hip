@@ -319,16 +319,21 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 }
                 if (lane == 0) candk[par * NWAVE + wave] = wkey;
                 __syncthreads();
+                // every wavefront's key and candidate pivot in one LDS round
+                // (a losing wavefront's entry may be stale; it is never used)
                 unsigned long long gk = candk[par * NWAVE];
+                double gv = cand[(par * NWAVE) * RB + jj];
                 int gw = 0;
+#pragma unroll
                 for (int q = 1; q < NWAVE; ++q) {
                     const unsigned long long k2 = candk[par * NWAVE + q];
-                    if (k2 > gk) { gk = k2; gw = q; }
+                    const double v2 = cand[(par * NWAVE + q) * RB + jj];
+                    if (k2 > gk) { gk = k2; gv = v2; gw = q; }
                 }
                 const int p = gk ? 4095 - (int)(gk & 0xFFFull) : 0x7fffffff;
                 // the pivot row, read by every lane at the same LDS address
                 const double* prow = cand + (par * NWAVE + gw) * RB;
-                const double piv = (p < N) ? prow[jj] : 0.0;
+                const double piv = (p < N) ? gv : 0.0;
                 if (!(fabs(piv) > 0.0) || !isfinite(piv)) {
                     singular = true;
                 } else {
