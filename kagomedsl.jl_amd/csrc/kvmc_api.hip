@@ -377,11 +377,12 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     ALLOC(h->rlist, nw); ALLOC(h->rcount, 1); ALLOC(h->failed, 1); ALLOC(h->all_walkers, nw);
     ALLOC(h->counters, (size_t)CNT_SLOTS);
     ALLOC(h->logdet, 2 * nw); ALLOC(h->eloc, nw); ALLOC(h->drift, 2);
-    // refresh workspace: up to 4 GiB of FP64 work matrices per batch (about
-    // 2300 walkers at 432 sites, so one launch pair covers a typical refresh)
+    // refresh workspace: up to 16 GiB of FP64 work matrices per batch (about
+    // 10000 walkers at 432 sites, so one launch pair covers even a refresh of
+    // every walker)
     {
         const size_t per_job = refresh_ws_doubles_per_job(h->ns, h->nmax) * sizeof(double);
-        size_t jobs = std::max<size_t>(2, ((size_t)4 << 30) / per_job);
+        size_t jobs = std::max<size_t>(2, ((size_t)16 << 30) / per_job);
         jobs = std::min(jobs, 2 * nw);
         h->ws_jobs = (int)(jobs & ~(size_t)1);
         if (h->ws_jobs < 2) h->ws_jobs = 2;
