@@ -42,7 +42,7 @@ FP64_PEAK_TFLOPS = 78.6
 # walker refreshed by the kernel pair k_refresh_blocked + k_refresh_solve, from
 # rocprofv3 --pmc passes over this script's default FP32 run (summaries in
 # profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_127_534}
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_155_310}
 # the same for k_sweep, per walker-sweep (FP32, delay 12, L=12)
 SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 18_014}
 
