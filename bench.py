@@ -7,18 +7,24 @@ N = 216 sweeps of every walker (one reference sweep = one spin-exchange
 proposal) followed by one local-energy measurement of every walker, i.e. one
 measurement interval of the reference cadence.
 
-Two forms run from identical seeds:
-  fp32  W in FP32 with `delay` pending rank-1 terms per flush, FP64 local
-        energy, FP64 LU refresh every k accepted moves  -> `value`
-  fp64  W in FP64, immediate rank-1 update and the refresh cadence of the
-        reference (src/MonteCarlo.jl:593-604: every min(N_up, N_dn) sweeps
-        that pass the early returns)
-`value` is the FP32 form's walker-sweeps/s.  Beside it: the FP64 form and the
-ratio, the FP32 forms at other refresh intervals, and |dE/site|, the
-difference of the mean local energy computed from the W the FP32 chains carry
-and from a fresh FP64 refresh of the same configurations (worst of several
-untimed steps after the timed region).  `roofline` covers
-whichever of the sweep and refresh kernels takes longer in the FP32 form.
+Forms (each its own handle):
+  fp32       W in FP32 with `delay` pending rank-1 terms per flush, FP64 local
+             energy, FP64 LU refresh every k accepted moves (EVERY_K: a walker
+             is refreshed as soon as it reaches k)            -> `value`
+  fp64       the plain FP64 sampler: W in FP64, immediate rank-1 updates,
+             FP64 refresh every N accepted moves (EVERY_K, k = N)
+  fp64_ref   FP64 with the reference's refresh cadence (src/MonteCarlo.jl:
+             593-604: every min(N_up, N_dn) sweeps that pass the early returns)
+  fp32 k=... the FP32 form at the other refresh intervals
+
+Accuracy: fp32 and fp64 start from the same configurations (drawn once),
+use the same Philox key, the same refresh interval and the same refresh
+phases, so their chains take identical decisions except at near ties.  After
+`--therm` thermalisation steps and the warm-up steps, every timed and probe
+step pairs each walker's FP32 local energy with its FP64 one; the line
+reports the mean |dE/site| with its standard error over walkers, the
+fraction of walkers whose configurations left the FP64 chain's, and the
+relative W drift just before every refresh of the FP32 form (probe steps).
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one
 GPU.
@@ -94,29 +100,49 @@ def refresh_min_bytes(ns, n, s):
     return 8 * ns * n + 4 * ns + s * (ns - n) * n + 4 * ns
 
 
-def run_form(nat, ham, args, precision, delay, k, label):
-    ns, n = ham.ns, ham.N_up
-    nw = args.walkers
-    S = args.sweeps_per_step or n
-    policy = ({"every_k": nat.REFRESH_EVERY_K, "at_end": nat.REFRESH_AT_END}[args.policy]
-              if k > 0 else nat.REFRESH_REFERENCE)
-    ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, precision=precision,
-                           refresh_policy=policy, refresh_k=max(k, 0), delay=delay,
-                           threads=args.threads, seed=args.seed)
+def make_sampler(nat, ham, args, precision, delay, k, policy_name):
+    """One form's handle: k <= 0 selects the reference refresh cadence."""
+    policy = (nat.REFRESH_REFERENCE if k <= 0 else
+              {"every_k": nat.REFRESH_EVERY_K, "at_end": nat.REFRESH_AT_END}[policy_name])
+    return nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, args.walkers, precision=precision,
+                             refresh_policy=policy, refresh_k=max(k, 0), delay=delay,
+                             threads=args.threads, seed=args.seed)
+
+
+def prepare(ds, args, label, configs=None):
+    """Seeded uniform starts (or the given configurations), then refresh
+    phases spread as in a long run, so that any timed window holds the
+    steady-state refresh rate of an every-k policy."""
     t0 = time.perf_counter()
-    redraws = ds.init_random(args.seed)
-    log(f"[{label}] init {nw} walkers L={args.L}: {time.perf_counter() - t0:.2f}s "
-        f"(redraws {redraws})")
-    # refresh phases spread as in a long run (every-k policy), so that any
-    # timed window holds the steady-state refresh rate
+    if configs is None:
+        redraws = ds.init_random(args.seed)
+    else:
+        ds.set_configs(*configs)
+        redraws = 0
     ds.stagger_refresh(args.seed)
     if args.graph:
         ds.enable_graph(True)
-    for w in range(args.warmup):
+    log(f"[{label}] start {args.walkers} walkers L={args.L}: {time.perf_counter() - t0:.2f}s "
+        f"(redraws {redraws})")
+    return redraws
+
+
+def untimed_steps(ds, S, nsteps, label, what):
+    for w in range(nsteps):
         t0 = time.perf_counter()
         ds.sweep(S)
         ds.measure()
-        log(f"[{label}] warmup step {w}: {time.perf_counter() - t0:.3f}s")
+        if w == nsteps - 1 or w < 2:
+            log(f"[{label}] {what} step {w}: {time.perf_counter() - t0:.3f}s")
+
+
+def timed_steps(nat, ham, ds, args, precision, label):
+    """K timed steps (S sweeps + one local-energy pass of every walker),
+    bracketed by device synchronisation; kernel times from the handle's own
+    HIP events.  Returns the summary and the per-step local energies."""
+    ns, n = ham.ns, ham.N_up
+    nw = args.walkers
+    S = args.sweeps_per_step or n
     sync()
     # graph replay needs profiling off: kernel times are then not split
     ds.profile(not args.graph)
@@ -125,7 +151,7 @@ def run_form(nat, ham, args, precision, delay, k, label):
     energies = []
     sync()
     t0 = time.perf_counter()
-    for st in range(args.steps):
+    for _ in range(args.steps):
         cnt, _ = ds.sweep(S)
         e, na = ds.measure()
         for key in tot:
@@ -136,28 +162,12 @@ def run_form(nat, ham, args, precision, delay, k, label):
     wall = time.perf_counter() - t0
     prof = ds.profile()
     ds.profile(False)
-    drift = ds.w_drift() if args.drift else None
-    # Accuracy of the carried W (outside the timed region): after each of
-    # `probe_steps` further steps, local energies from the W the chains hold
-    # against those of a fresh FP64 refresh of the same configurations (a
-    # second handle); the worst step is reported.
-    d_mean = d_max = 0.0
-    if precision != nat.PREC_FP64 and args.probe_steps > 0:
-        ref = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, precision=nat.PREC_FP64)
-        for _ in range(args.probe_steps):
-            ds.sweep(S)
-            e_carried, _ = ds.measure()
-            ref.set_configs(*ds.get_configs())
-            e_fresh, _ = ref.measure()
-            d_mean = max(d_mean, abs(float(np.mean(e_carried) - np.mean(e_fresh))) / ns)
-            d_max = max(d_max, float(np.abs(e_carried - e_fresh).max()) / ns)
-        ref.close()
-    ds.close()
     s = 8 if precision == nat.PREC_FP64 else 4
     b_sweep = sweep_bytes(tot, s, ns, n, ham.N_down, nw)
     b_eloc = s * 2 * nap + 8 * ns * nw * args.steps
     flops_walker = refresh_flops(ns, n) + refresh_flops(ns, ham.N_down)
     flops_ref = tot["refreshes"] * flops_walker
+
     def rate(num, ms):
         return num / (ms * 1e-3) if ms > 0 else None
 
@@ -181,16 +191,99 @@ def run_form(nat, ham, args, precision, delay, k, label):
         "sweep_ms_per_launch": prof["sweep_ms"] / max(prof["sweep_launches"], 1),
         "sweep_bytes_per_launch": b_sweep / max(prof["sweep_launches"], 1),
         "walker_sweeps_per_launch": nw * S * args.steps / max(prof["sweep_launches"], 1),
+        "accepts_per_launch": tot["accepted"] / max(prof["sweep_launches"], 1),
         "energy_per_site": float(np.mean(energies) / ns),
-        "w_drift": drift,
-        "dE_per_site_carried_vs_refreshed": d_mean,
-        "max_walker_dE_per_site": d_max,
     }
     log(f"[{label}] {res['walker_sweeps_per_s']:.3e} walker-sweeps/s, acc {res['accept_rate']:.3f}, "
-        f"sweep {res['sweep_gbs']:.0f} GB/s, kernel ms {res['kernel_ms']}, refreshes "
-        f"{tot['refreshes']}, dE/site {res['dE_per_site_carried_vs_refreshed']:.2e} "
-        f"(max walker {res['max_walker_dE_per_site']:.2e})")
+        f"sweep {res['sweep_gbs']:.0f} GB/s, kernel ms {res['kernel_ms']}, refreshes {tot['refreshes']}")
     return res, np.array(energies)
+
+
+def run_form(nat, ham, args, precision, delay, k, policy_name, label, warmup=None):
+    """A form timed on its own (own starts): thermalisation-free warm-up,
+    then the timed steps."""
+    ds = make_sampler(nat, ham, args, precision, delay, k, policy_name)
+    prepare(ds, args, label)
+    untimed_steps(ds, args.sweeps_per_step or ham.N_up, args.warmup if warmup is None else warmup,
+                  label, "warmup")
+    res, e = timed_steps(nat, ham, ds, args, precision, label)
+    ds.close()
+    return res, e
+
+
+def paired_accuracy(nat, ham, args, ds32, ds64, e32, e64):
+    """Paired FP32-vs-FP64 local energies over the timed steps (e32, e64:
+    steps x walkers) and `probe_steps` further steps, in which the FP32
+    handle also records its relative W drift just before every refresh and
+    the carried FP32 W is checked against a fresh FP64 refresh of the same
+    configurations (a third handle)."""
+    ns = ham.ns
+    S = args.sweeps_per_step or ham.N_up
+    d_steps = [(a - b) / ns for a, b in zip(e32, e64)]
+    diverged = np.zeros(args.walkers, dtype=bool)
+    carried_mean = carried_max = 0.0
+    ds32.track_drift(True)
+    ref = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, args.walkers, precision=nat.PREC_FP64)
+    for st in range(args.probe_steps):
+        ds32.sweep(S)
+        a, _ = ds32.measure()
+        ds64.sweep(S)
+        b, _ = ds64.measure()
+        d_steps.append((a - b) / ns)
+        k32, k64 = ds32.get_configs(), ds64.get_configs()
+        diverged |= np.any(k32[0] != k64[0], axis=1) | np.any(k32[1] != k64[1], axis=1)
+        ref.set_configs(*k32)
+        e_fresh, _ = ref.measure()
+        carried_mean = max(carried_mean, abs(float(np.mean(a) - np.mean(e_fresh))) / ns)
+        carried_max = max(carried_max, float(np.abs(a - e_fresh).max()) / ns)
+        log(f"[accuracy] probe step {st}: mean dE/site {np.mean(d_steps[-1]):+.2e}, "
+            f"diverged walkers {int(diverged.sum())}")
+    ref.close()
+    dmax, dmean, dcount = ds32.drift_stats()
+    ds32.track_drift(False)
+    D = np.stack(d_steps)                    # steps x walkers, per site
+    per_walker = D.mean(axis=0)
+    mean = float(per_walker.mean())
+    stderr = float(per_walker.std(ddof=1) / np.sqrt(len(per_walker)))
+    same = ~diverged
+    out = {
+        "dE_per_site_mean": mean,
+        "dE_per_site_stderr": stderr,
+        "abs_dE_per_site": abs(mean),
+        "within_1e-5": bool(abs(mean) + 2 * stderr <= 1e-5),
+        "steps_paired": int(D.shape[0]),
+        "walkers": int(D.shape[1]),
+        "diverged_fraction": float(diverged.mean()),
+        "identical_walkers_dE_per_site_mean": float(per_walker[same].mean()) if same.any() else None,
+        "identical_walkers_max_abs_dE_per_site": float(np.abs(D[:, same]).max()) if same.any() else None,
+        "w_drift_before_refresh_max": dmax,
+        "w_drift_before_refresh_mean": dmean,
+        "w_drift_refreshes": int(dcount),
+        "carried_vs_refreshed_dE_per_site": carried_mean,
+        "carried_vs_refreshed_max_walker_dE_per_site": carried_max,
+    }
+    log(f"[accuracy] dE/site {mean:+.2e} +- {stderr:.1e} over {D.shape[1]} walkers x {D.shape[0]} steps; "
+        f"diverged {out['diverged_fraction']:.4f}; W drift before refresh max {dmax:.2e} "
+        f"mean {dmean:.2e} over {dcount} refreshes")
+    return out
+
+
+def host_cpu():
+    """Logical CPUs of the host, the ones this process may use, and the model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return os.cpu_count() or 1, usable, model
 
 
 def cpu_baseline(ham, seconds, threads, seed):
@@ -249,17 +342,28 @@ def roofline_refresh(r, ns, traffic_per_walker):
             "units_per_launch": wpl, "launch_ms": r["refresh_ms_per_launch"]}
 
 
-def roofline_sweep(r, traffic_per_walker_sweep):
+def roofline_sweep(r, traffic_per_walker_sweep, ham, s=4):
+    """HBM roofline of the sweep kernel.  `achieved` counts the bytes the
+    algorithm as built must move (flushes of the empty-site rows of W once
+    per `delay` accepts, the column and row of each accept, two gathers per
+    move; sweep_bytes).  Beside it, the per-accept unit (a full read and
+    write of both ns x N W per accepted move, 2 s ns^2, plus 2 s + 16 per
+    proposal): the work an immediate rank-1 update of the full W would have
+    to stream, delivered in the same time."""
     if not r["sweep_ms_per_launch"]:
         return None
-    achieved = r["sweep_bytes_per_launch"] / (r["sweep_ms_per_launch"] * 1e-3) / 1e9
+    t = r["sweep_ms_per_launch"] * 1e-3
+    achieved = r["sweep_bytes_per_launch"] / t / 1e9
     wsl = r["walker_sweeps_per_launch"]
+    per_accept = (r["accepts_per_launch"] * 2 * s * ham.ns * ham.ns + wsl * (2 * s + 16)) / t / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_per_walker_sweep * wsl if traffic_per_walker_sweep else None,
             "units_per_launch": wsl,
             "kernel": "k_sweep (exchange step + delayed Sherman-Morrison updates)",
-            "launch_ms": r["sweep_ms_per_launch"]}
+            "launch_ms": r["sweep_ms_per_launch"],
+            "achieved_per_accept_unit": per_accept,
+            "frac_per_accept_unit": per_accept / HBM_PEAK_GBS}
 
 
 MIXED_GROUPS = ((4, 4096), (6, 2048), (8, 2048), (10, 1024), (12, 1024))
@@ -378,16 +482,19 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="sweep-kernel workgroup size (0: automatic)")
     ap.add_argument("--k", type=int, default=216, help="FP32 form: accepted moves per refresh")
     ap.add_argument("--k64", type=int, default=0,
-                    help="FP64 form: accepted moves per refresh (0: the reference cadence)")
-    ap.add_argument("--policy", default="at_end", choices=("every_k", "at_end"),
-                    help="FP32 forms: refresh mid-call (every_k) or after the call (at_end)")
-    ap.add_argument("--k-extra", default="16,64,128",
-                    help="further FP32 refresh intervals reported beside the headline")
+                    help="plain FP64 form: accepted moves per refresh (0: N, one per electron)")
+    ap.add_argument("--policy", default="every_k", choices=("every_k", "at_end"),
+                    help="FP32 headline: refresh mid-call (every_k) or after the call (at_end)")
+    ap.add_argument("--k-extra", default="16,64,at_end:16,at_end:216",
+                    help="further FP32 forms beside the headline: k (every-k policy) or at_end:k")
+    ap.add_argument("--therm", type=int, default=20,
+                    help="thermalisation steps of the paired fp32/fp64 forms before the warm-up")
+    ap.add_argument("--skip-fp64-ref", action="store_true",
+                    help="skip the FP64 form with the reference refresh cadence")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--drift", action="store_true")
     ap.add_argument("--probe-steps", type=int, default=4,
                     help="untimed steps after the timed region, each checking the carried W")
     ap.add_argument("--graph", action="store_true",
@@ -424,30 +531,59 @@ def main():
     ham = Hamiltonian.build(n, n, lat)
     log(f"L={L} ns={ham.ns} nb={len(ham.nn)} walkers={args.walkers}")
 
-    r32, e32 = run_form(nat, ham, args, nat.PREC_FP32, args.delay, args.k, "fp32")
-    r64 = e64 = None
+    S = args.sweeps_per_step or n
+    k64 = args.k64 or n
+    # headline pair: FP32 and plain FP64 from the same configurations, Philox
+    # key, refresh interval and refresh phases
+    ds32 = make_sampler(nat, ham, args, nat.PREC_FP32, args.delay, args.k, args.policy)
+    prepare(ds32, args, "fp32")
+    ds64 = None
     if not args.skip_fp64:
-        r64, e64 = run_form(nat, ham, args, nat.PREC_FP64, 1, args.k64, "fp64")
+        ds64 = make_sampler(nat, ham, args, nat.PREC_FP64, 1, k64, "every_k")
+        prepare(ds64, args, "fp64", configs=ds32.get_configs())
+    for ds, label in ((ds32, "fp32"), (ds64, "fp64")):
+        if ds is not None:
+            untimed_steps(ds, S, args.therm + args.warmup, label, "thermalisation/warm-up")
+    r32, e32 = timed_steps(nat, ham, ds32, args, nat.PREC_FP32, "fp32")
+    r64 = e64 = acc = None
+    if ds64 is not None:
+        r64, e64 = timed_steps(nat, ham, ds64, args, nat.PREC_FP64, "fp64")
+        acc = paired_accuracy(nat, ham, args, ds32, ds64, e32, e64)
+        ds64.close()
+    ds32.close()
+    r64ref = None
+    if not args.skip_fp64 and not args.skip_fp64_ref:
+        r64ref, _ = run_form(nat, ham, args, nat.PREC_FP64, 1, 0, "every_k", "fp64 reference cadence")
     extra = {}
-    for k in [int(x) for x in args.k_extra.split(",") if x.strip()]:
-        rk, _ = run_form(nat, ham, args, nat.PREC_FP32, args.delay, k, f"fp32 k={k}")
-        extra[str(k)] = {key: rk[key] for key in (
-            "walker_sweeps_per_s", "ms_per_step", "kernel_ms", "refresh_tflops",
-            "dE_per_site_carried_vs_refreshed", "max_walker_dE_per_site")}
-        extra[str(k)]["refreshes"] = rk["counters"]["refreshes"]
+    for item in [x.strip() for x in args.k_extra.split(",") if x.strip()]:
+        pol, _, kk = item.rpartition(":")
+        pol = pol or "every_k"
+        k = int(kk)
+        rk, _ = run_form(nat, ham, args, nat.PREC_FP32, args.delay, k, pol, f"fp32 {pol} k={k}")
+        extra[f"{pol}:{k}"] = {key: rk[key] for key in (
+            "walker_sweeps_per_s", "ms_per_step", "kernel_ms", "refresh_tflops")}
+        extra[f"{pol}:{k}"]["refreshes"] = rk["counters"]["refreshes"]
+        if r64 is not None:
+            extra[f"{pol}:{k}"]["speedup_vs_fp64"] = rk["walker_sweeps_per_s"] / r64["walker_sweeps_per_s"]
 
     cpu = None
     if args.cpu_seconds > 0:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        nproc, usable, model = host_cpu()
+        # the GPU box allots 16 host threads to each GPU
+        threads = args.cpu_threads or min(usable, 16)
         rate, nsw, wall = cpu_baseline(ham, args.cpu_seconds, threads, args.seed)
         cpu = {"value": rate, "unit": "walker-sweeps/s", "cores": threads, "kind": "port",
+               "nproc": nproc, "usable_cpus": usable, "cpu_model": model,
+               "per_core": rate / threads,
                "sample": f"oracle C restatement (complex128 W, reference kernel and cadence), "
-                         f"L={L}, {threads} chains x {nsw // threads} sweeps in {wall:.1f}s"}
-        log(f"[cpu] {rate:.3e} walker-sweeps/s on {threads} threads")
+                         f"L={L}, {threads} independent chains (one per thread) x {nsw // threads} sweeps "
+                         f"in {wall:.1f}s"}
+        log(f"[cpu] {rate:.3e} walker-sweeps/s on {threads} threads ({nproc} logical CPUs, {model})")
 
     # the dominant kernel of the headline form carries the roofline
-    rr = roofline_refresh(r32, ham.ns, REFRESH_TRAFFIC_BYTES_PER_WALKER.get(L))
-    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if args.delay == 12 else None)
+    traffic_ok = args.delay == 12 and args.policy == "every_k" and args.k == 216
+    rr = roofline_refresh(r32, ham.ns, REFRESH_TRAFFIC_BYTES_PER_WALKER.get(L) if traffic_ok else None)
+    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if traffic_ok else None, ham)
     dominant = rr if rr and r32["kernel_ms"]["refresh_ms"] > r32["kernel_ms"]["sweep_ms"] else rs
     value = r32["walker_sweeps_per_s"]
     line = {
@@ -467,23 +603,26 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"pi-flux Kagome L={L} ({ham.ns} sites, {n}/{n}), {args.walkers} walkers, "
-                        f"step = {args.sweeps_per_step or n} sweeps + 1 E_loc; FP32 W, delay {args.delay}, "
-                        f"FP64 refresh every {args.k} accepts ({args.policy})",
+                        f"step = {S} sweeps + 1 E_loc; FP32 W, delay {args.delay}, "
+                        f"FP64 refresh every {args.k} accepts ({args.policy}); {args.therm} thermalisation "
+                        f"steps before the warm-up",
             "L": L, "walkers": args.walkers, "delay": args.delay, "refresh_k": args.k,
             "refresh_policy": args.policy,
         },
         "roofline": dominant,
         "roofline_other": rs if dominant is rr else rr,
         "cpu_baseline": cpu,
+        "accuracy_fp32_vs_fp64": acc,
         "fp32": r32,
         "fp64": r64,
-        "fp32_other_k": extra,
+        "fp64_reference_cadence": r64ref,
+        "fp32_other": extra,
     }
-    line["dE_per_site_fp32"] = r32["dE_per_site_carried_vs_refreshed"]
     if r64 is not None:
         line["fp64_walker_sweeps_per_s"] = r64["walker_sweeps_per_s"]
         line["speedup_fp32_vs_fp64"] = value / r64["walker_sweeps_per_s"]
-        line["dE_per_site_fp32_vs_fp64_chains"] = abs(float(np.mean(e32) - np.mean(e64))) / ham.ns
+    if r64ref is not None:
+        line["speedup_fp32_vs_fp64_reference_cadence"] = value / r64ref["walker_sweeps_per_s"]
     line["eloc_evals_per_s"] = r32["eloc_evals_per_s_kernel"]
     print(json.dumps(line), flush=True)
 

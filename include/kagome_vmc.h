@@ -185,9 +185,24 @@ int kvmc_enable_graph(kvmc_handle* h, int32_t enable);
  * counterpart. */
 int kvmc_stagger_refresh(kvmc_handle* h, uint64_t seed);
 
-/* Largest max|W - W_refreshed| / max|W_refreshed| over walkers, measured by
- * recomputing W in FP64 without replacing it (drift of the rank-1 updates). */
+/* Largest max|W - W_refreshed| / max|W_refreshed| over walkers (each
+ * walker's maxima over both spins), measured by recomputing W in FP64
+ * without replacing it (drift of the rank-1 updates).  Stands beside
+ * reevaluateW! (src/MonteCarlo.jl:55-66), which replaces W without
+ * measuring what it replaced. */
 int kvmc_w_drift(kvmc_handle* h, double* max_rel_drift);
+
+/* Enables (1) or disables (0) a drift pass before every refresh that the
+ * refresh policy triggers inside kvmc_sweep (src/MonteCarlo.jl:593-604):
+ * each refreshed walker's max|W - W_fresh| / max|W_fresh| is recorded just
+ * before W is replaced.  Costs one extra refresh computation per refreshed
+ * walker; graph replay is bypassed while enabled.  Resets the statistics. */
+int kvmc_track_drift(kvmc_handle* h, int32_t enable);
+
+/* Statistics of the drift passes since kvmc_track_drift: the largest and
+ * the mean relative drift over refreshed walkers, and their number.  Any
+ * output may be NULL. */
+int kvmc_get_drift_stats(kvmc_handle* h, double* max_rel, double* mean_rel, int64_t* count);
 
 /* Kernel timing measured with HIP events on the handle's stream (no
  * reference counterpart; used by benchmarks).  Enabling resets the sums. */

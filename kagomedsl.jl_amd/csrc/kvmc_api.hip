@@ -55,6 +55,22 @@ int dalloc(T** p, size_t count) {
     return KVMC_OK;
 }
 
+// Device buffer owned by a scope: freed on every exit path.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    int alloc(size_t count) { reset(); return dalloc(&p, count); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    T* release() { T* q = p; p = nullptr; return q; }
+};
+
 // Seeded uniform Sz=0 configuration (Fisher-Yates over Philox words keyed
 // (seed, walker, attempt)).
 void random_config(uint64_t seed, uint32_t walker, uint32_t attempt, int ns, int nup, int ndn,
@@ -102,6 +118,28 @@ __global__ void k_fill_ull(unsigned long long* p, unsigned long long v, int n) {
     if (i < n) p[i] = v;
 }
 
+// Relative drift of each listed walker (max |W - W_fresh| / max |W_fresh|
+// over both spins, from the per-walker maxima of a drift pass) folded into
+// stats[0] (maximum, ordered bits), stats[1] (sum) and stats[2] (count).
+__global__ void k_drift_fold(const int* list, int count, const double* drift, double* stats) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int w = list[i];
+    const double d = drift[(size_t)w * 2], v = drift[(size_t)w * 2 + 1];
+    const double rel = v > 0.0 ? d / v : 0.0;
+    atomicMax(reinterpret_cast<unsigned long long*>(stats), (unsigned long long)__double_as_longlong(rel));
+    atomicAdd(&stats[1], rel);
+    atomicAdd(&stats[2], 1.0);
+}
+
+// Clears the per-walker drift maxima of the listed walkers.
+__global__ void k_drift_clear(const int* list, int count, double* drift) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    drift[(size_t)list[i] * 2] = 0.0;
+    drift[(size_t)list[i] * 2 + 1] = 0.0;
+}
+
 // Sweep launches per replay of a captured sweep sequence: sweep, refresh
 // the walkers that paused, sweep again, ...; walkers still paused after the
 // last round are finished by the host loop.
@@ -146,7 +184,9 @@ struct kvmc_handle {
     unsigned long long* counters = nullptr;
     double* logdet = nullptr;
     double* eloc = nullptr;
-    double* drift = nullptr;
+    double* drift = nullptr;        // [nw][2] per-walker drift maxima of a drift pass
+    double* dstats = nullptr;       // [3] drift before refresh: max, sum, count
+    bool track_drift = false;       // drift pass before every policy refresh
     double* ws = nullptr;
     int ws_jobs = 0;
     int nparams = 0;
@@ -169,7 +209,7 @@ void free_all(kvmc_handle* h) {
     void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->kup, h->kdn, h->W, h->slot, h->A,
                     h->Bm, h->npend, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
                     h->tbase, h->rlist, h->rcount, h->failed, h->all_walkers, h->counters,
-                    h->logdet, h->eloc, h->drift, h->ws, h->Qup, h->Qdn, h->O};
+                    h->logdet, h->eloc, h->drift, h->dstats, h->ws, h->Qup, h->Qdn, h->O};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -222,12 +262,29 @@ RefreshArgs refresh_args(const kvmc_handle* h, int write_w, double pivot_tol) {
 }
 
 int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
-                 double pivot_tol = 0.0) {
+                 double pivot_tol = 0.0, bool drift_first = false) {
     if (count <= 0) return KVMC_OK;
     static const int kNone = INT_MAX;
     HIP_TRY(hipMemcpyAsync(h->failed, &kNone, sizeof(int), hipMemcpyHostToDevice, h->stream));
-    RefreshArgs r = refresh_args(h, write_w, pivot_tol);
     const int batch_walkers = std::max(1, h->ws_jobs / 2);
+    const int fb = (count + 255) / 256;
+    if (drift_first || !write_w) {
+        hipLaunchKernelGGL(k_drift_clear, dim3(fb), dim3(256), 0, h->stream, d_list, count, h->drift);
+        HIP_TRY(hipGetLastError());
+    }
+    if (drift_first) {
+        // the carried W against a fresh FP64 refresh, just before it is replaced
+        RefreshArgs d = refresh_args(h, 0, pivot_tol);
+        for (int b0 = 0; b0 < count; b0 += batch_walkers) {
+            const int nwb = std::min(batch_walkers, count - b0);
+            d.jobs = d_list + b0;
+            d.njobs = 2 * nwb;
+            HIP_TRY(launch_refresh(h->prec, d, h->stream));
+        }
+        hipLaunchKernelGGL(k_drift_fold, dim3(fb), dim3(256), 0, h->stream, d_list, count, h->drift, h->dstats);
+        HIP_TRY(hipGetLastError());
+    }
+    RefreshArgs r = refresh_args(h, write_w, pivot_tol);
     prof_begin(h);
     for (int b0 = 0; b0 < count; b0 += batch_walkers) {
         const int nwb = std::min(batch_walkers, count - b0);
@@ -376,7 +433,7 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     ALLOC(h->sweeps, nw); ALLOC(h->stop, nw); ALLOC(h->tbase, nw);
     ALLOC(h->rlist, nw); ALLOC(h->rcount, 1); ALLOC(h->failed, 1); ALLOC(h->all_walkers, nw);
     ALLOC(h->counters, (size_t)CNT_SLOTS);
-    ALLOC(h->logdet, 2 * nw); ALLOC(h->eloc, nw); ALLOC(h->drift, 2);
+    ALLOC(h->logdet, 2 * nw); ALLOC(h->eloc, nw); ALLOC(h->drift, 2 * nw); ALLOC(h->dstats, 3);
     // refresh workspace: up to 16 GiB of FP64 work matrices per batch (about
     // 10000 walkers at 432 sites, so one launch pair covers even a refresh of
     // every walker)
@@ -424,6 +481,7 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         hipMemsetAsync(h->npend, 0, nw * sizeof(int), h->stream) != hipSuccess ||
         hipMemsetAsync(h->acc_since, 0, nw * sizeof(int), h->stream) != hipSuccess ||
         hipMemsetAsync(h->logdet, 0, 2 * nw * sizeof(double), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->dstats, 0, 3 * sizeof(double), h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess) {
         free_all(h);
         delete h;
@@ -498,14 +556,13 @@ int kvmc_init_random(kvmc_handle* h, uint64_t seed, int32_t max_attempts, int64_
         HIP_TRY(hipMemcpyAsync(h->kup, ku.data(), ku.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
         HIP_TRY(hipMemcpyAsync(h->kdn, kd.data(), kd.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
         nredraw += (int64_t)bad.size();
-        int* dbad = nullptr;
-        if ((rc = dalloc(&dbad, bad.size())) != KVMC_OK) return rc;
-        HIP_TRY(hipMemcpyAsync(dbad, bad.data(), bad.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        DevBuf<int> dbad;
+        if ((rc = dbad.alloc(bad.size())) != KVMC_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(dbad.p, bad.data(), bad.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
         for (int w : bad) status[w] = 0;
         HIP_TRY(hipMemcpyAsync(h->status, status.data(), sizeof(int) * h->nw, hipMemcpyHostToDevice, h->stream));
-        rc = refresh_list(h, dbad, (int)bad.size(), 1, start_tol);
+        rc = refresh_list(h, dbad.p, (int)bad.size(), 1, start_tol);
         HIP_TRY(hipStreamSynchronize(h->stream));
-        (void)hipFree(dbad);
         if (rc != KVMC_OK && rc != KVMC_ERR_SINGULAR) return rc;
     }
     if (redraws) *redraws = nredraw;
@@ -562,7 +619,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
     if (lds > 160 * 1024) { cleanup(); return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay"); }
     int64_t launches = 0;
     hipStream_t s = h->stream;
-    const bool use_graph = h->graph && !tape_u && !outcomes && !h->profiling;
+    const bool use_graph = h->graph && !tape_u && !outcomes && !h->profiling && !h->track_drift;
     bool finished = false;   // the replay left no walker paused
     if (use_graph) {
         // One replay: stop counts, counter and flag resets (fill kernels, no
@@ -619,7 +676,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         if (count == 0) {
             finished = true;
         } else {
-            rc = refresh_list(h, h->rlist, count, 1);
+            rc = refresh_list(h, h->rlist, count, 1, 0.0, h->track_drift);
             if (rc) { cleanup(); return rc; }
             finished = h->policy == KVMC_REFRESH_AT_END;
         }
@@ -646,7 +703,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         h->prof.sweep_ms += prof_elapsed(h);
         h->prof.sweep_launches += h->profiling;
         if (count == 0) break;
-        rc = refresh_list(h, h->rlist, count, 1);
+        rc = refresh_list(h, h->rlist, count, 1, 0.0, h->track_drift);
         if (rc) { cleanup(); return rc; }
         if (h->policy == KVMC_REFRESH_AT_END) break;   // listed walkers had finished
     }
@@ -720,17 +777,21 @@ int kvmc_set_orbital_derivs(kvmc_handle* h, int32_t nparams, const double* du_up
     if (nparams == 0) return KVMC_OK;
     const size_t ns = h->ns;
     int rc;
-    double *dU = nullptr;
-    if ((rc = dalloc(&h->Qup, (size_t)nparams * ns * ns)) || (rc = dalloc(&h->Qdn, (size_t)nparams * ns * ns)) ||
-        (rc = dalloc(&h->O, (size_t)nparams * h->nw)) || (rc = dalloc(&dU, (size_t)nparams * ns * h->nmax)))
+    // built in scope-owned buffers; the handle takes them only on success
+    DevBuf<double> qu, qd, o, dU;
+    if ((rc = qu.alloc((size_t)nparams * ns * ns)) || (rc = qd.alloc((size_t)nparams * ns * ns)) ||
+        (rc = o.alloc((size_t)nparams * h->nw)) || (rc = dU.alloc((size_t)nparams * ns * h->nmax)))
         return rc;
     hipStream_t s = h->stream;
-    HIP_TRY(hipMemcpyAsync(dU, du_up, (size_t)nparams * ns * h->nup * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_q_build(dU, h->Uup, h->ns, h->nup, nparams, h->Qup, s));
-    HIP_TRY(hipMemcpyAsync(dU, du_dn, (size_t)nparams * ns * h->ndn * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_q_build(dU, h->Udn, h->ns, h->ndn, nparams, h->Qdn, s));
+    HIP_TRY(hipMemcpyAsync(dU.p, du_up, (size_t)nparams * ns * h->nup * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_q_build(dU.p, h->Uup, h->ns, h->nup, nparams, qu.p, s));
     HIP_TRY(hipStreamSynchronize(s));
-    (void)hipFree(dU);
+    HIP_TRY(hipMemcpyAsync(dU.p, du_dn, (size_t)nparams * ns * h->ndn * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_q_build(dU.p, h->Udn, h->ns, h->ndn, nparams, qd.p, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    h->Qup = qu.release();
+    h->Qdn = qd.release();
+    h->O = o.release();
     h->nparams = nparams;
     return KVMC_OK;
 }
@@ -863,12 +924,41 @@ int kvmc_w_drift(kvmc_handle* h, double* max_rel) {
     g_err.clear();
     if (!h || !max_rel) return fail(KVMC_ERR_INVALID, "null argument");
     if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
-    HIP_TRY(hipMemsetAsync(h->drift, 0, 2 * sizeof(double), h->stream));
     int rc = refresh_all(h, 0);
     if (rc) return rc;
-    double dr[2] = {0, 0};
-    HIP_TRY(hipMemcpy(dr, h->drift, sizeof(dr), hipMemcpyDeviceToHost));
-    *max_rel = dr[1] > 0 ? dr[0] / dr[1] : 0.0;
+    std::vector<double> dr((size_t)2 * h->nw);
+    HIP_TRY(hipMemcpyAsync(dr.data(), h->drift, dr.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double m = 0.0;
+    for (int w = 0; w < h->nw; ++w)
+        if (dr[2 * w + 1] > 0.0) m = std::max(m, dr[2 * w] / dr[2 * w + 1]);
+    *max_rel = m;
+    return KVMC_OK;
+}
+
+int kvmc_track_drift(kvmc_handle* h, int32_t enable) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    h->track_drift = enable != 0;
+    if (h->gexec) {   // a captured sequence would skip the drift passes
+        HIP_TRY(hipGraphExecDestroy(h->gexec));
+        h->gexec = nullptr;
+        h->g_nsweeps = -1;
+    }
+    HIP_TRY(hipMemsetAsync(h->dstats, 0, 3 * sizeof(double), h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return KVMC_OK;
+}
+
+int kvmc_get_drift_stats(kvmc_handle* h, double* max_rel, double* mean_rel, int64_t* count) {
+    g_err.clear();
+    if (!h) return fail(KVMC_ERR_INVALID, "null handle");
+    double st[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(st, h->dstats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (max_rel) *max_rel = st[0];
+    if (mean_rel) *mean_rel = st[2] > 0 ? st[1] / st[2] : 0.0;
+    if (count) *count = (int64_t)st[2];
     return KVMC_OK;
 }
 

@@ -127,7 +127,7 @@ struct RefreshArgs {
     int* failed;                    // smallest failing walker (atomicMin)
     int write_w;                    // 1: replace W; 0: measure drift only
     double pivot_tol;               // reject when min|pivot| < pivot_tol * max|pivot|
-    double* drift;                  // [2]: max |W_new - W_cur|, max |W_new| (as ordered bits)
+    double* drift;                  // [nw][2] per walker: max |W_new - W_cur|, max |W_new| (ordered bits)
 };
 
 struct ElocArgs {

@@ -647,8 +647,8 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     if (a.write_w) {
         for (int s = t; s < ns; s += RT) sl[s] = kap[s] == 0 ? rowof[s] - N : -1;
     } else {
-        atomic_max_nonneg2(&a.drift[0], dmax);
-        atomic_max_nonneg2(&a.drift[1], vmax);
+        atomic_max_nonneg2(&a.drift[(size_t)w * 2], dmax);
+        atomic_max_nonneg2(&a.drift[(size_t)w * 2 + 1], vmax);
     }
 }
 
@@ -894,8 +894,8 @@ k_refresh_solve(RefreshArgs a, int groups) {
             if (kap[s] != 0) sl[s] = -1;
         if (t < 64 && 64 * grp + t < nE) sl[esite[64 * grp + t]] = 64 * grp + t;
     } else {
-        atomic_max_nonneg2(&a.drift[0], dmax);
-        atomic_max_nonneg2(&a.drift[1], vmax);
+        atomic_max_nonneg2(&a.drift[(size_t)w * 2], dmax);
+        atomic_max_nonneg2(&a.drift[(size_t)w * 2 + 1], vmax);
     }
 }
 

@@ -25,7 +25,7 @@ EXPORTS = [
     "kvmc_measure_grad", "kvmc_get_logdet", "kvmc_get_w", "kvmc_set_sweep_count",
     "kvmc_get_sweep_count", "kvmc_w_drift", "kvmc_failed_walker", "kvmc_last_error",
     "kvmc_version", "kvmc_profile_enable", "kvmc_get_profile", "kvmc_stagger_refresh",
-    "kvmc_enable_graph",
+    "kvmc_enable_graph", "kvmc_track_drift", "kvmc_get_drift_stats",
 ]
 
 
@@ -103,6 +103,8 @@ def load(path: str = LIB_PATH):
         "kvmc_get_profile": ([vp, P(KvmcProfile)], C.c_int),
         "kvmc_stagger_refresh": ([vp, C.c_uint64], C.c_int),
         "kvmc_enable_graph": ([vp, C.c_int32], C.c_int),
+        "kvmc_track_drift": ([vp, C.c_int32], C.c_int),
+        "kvmc_get_drift_stats": ([vp, P(C.c_double), P(C.c_double), P(C.c_int64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -261,6 +263,18 @@ class DeviceSampler:
         v = C.c_double(0.0)
         _check(load().kvmc_w_drift(self._h, C.byref(v)))
         return v.value
+
+    def track_drift(self, enable=True):
+        """Record each walker's relative W drift just before every policy
+        refresh (resets the statistics)."""
+        _check(load().kvmc_track_drift(self._h, int(bool(enable))))
+
+    def drift_stats(self):
+        """(max, mean, count) of the relative W drift before refresh since
+        track_drift was enabled."""
+        mx, mn, n = C.c_double(0.0), C.c_double(0.0), C.c_int64(0)
+        _check(load().kvmc_get_drift_stats(self._h, C.byref(mx), C.byref(mn), C.byref(n)))
+        return mx.value, mn.value, n.value
 
     def profile(self, enable=None):
         """With enable=True/False: (re)start or stop kernel timing.  Without:

@@ -118,8 +118,8 @@ def test_energy_gradcheck_device():
         return exhaustive_average(f, th, 6, 6)
 
     theta = torch.tensor(THETA1, dtype=torch.float64, requires_grad=True)
-    assert torch.autograd.gradcheck(lambda t: energy(t, evaluate), (theta,), eps=1e-5, atol=1e-6,
-                                    rtol=1e-5)
+    assert torch.autograd.gradcheck(lambda t: energy(t, evaluate), (theta,), eps=1e-5, atol=1e-9,
+                                    rtol=1e-6)
 
 
 def _jackknife(E, O, nblocks=32):
