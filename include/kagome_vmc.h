@@ -42,6 +42,16 @@ extern "C" {
 #define KVMC_PREC_FP64 0
 #define KVMC_PREC_FP32 1
 
+/* Orbital type.  REAL: u_up / u_down hold ns x N doubles (the pi-flux ansatz
+ * at B = 0 is real, and so are U and W).  COMPLEX: they hold ns x N complex
+ * doubles, interleaved (re, im) like Julia's Matrix{ComplexF64}: flux phases
+ * or a magnetic field make H complex (src/Hamiltonian.jl:201, 325), U comes
+ * from eigen(Hermitian) (:382-393) and W has eltype(U) (src/MonteCarlo.jl:
+ * 181-182), updated by a complex geru! (:290).  W is then stored as complex
+ * float (FP32) or complex double (FP64); the acceptance uses |ratio|^2. */
+#define KVMC_ORB_REAL 0
+#define KVMC_ORB_COMPLEX 1
+
 /* Transition kernel.  REFERENCE: the reference's shared-uniform kernel
  * (src/MonteCarlo.jl:538-607), stationary law |psi|^2 / Z_mu.
  * UNBIASED: uniform bond, accept with min(1, |ratio|^2), law |psi|^2. */
@@ -84,6 +94,7 @@ typedef struct kvmc_desc {
                                rank-`delay` flush of W (<= 1: immediate) */
     int32_t threads;        /* threads per workgroup of the sweep kernel (0: auto) */
     uint64_t seed;          /* Philox key of the device RNG */
+    int32_t orbitals;       /* KVMC_ORB_* */
 } kvmc_desc;
 
 typedef struct kvmc_counters {
@@ -142,24 +153,30 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
 int kvmc_measure(kvmc_handle* h, double* eloc, double* sum, int64_t* n_antiparallel);
 
 /* Sets ns x N column-major derivatives dU_k/dtheta_k of the occupied
- * orbitals for nparams variational parameters (nparams*ns*N doubles per
- * spin, parameter-major).  Enables kvmc_measure_grad.  No reference
- * counterpart: the reference has no differentiation. */
+ * orbitals for nparams variational parameters (nparams*ns*N values per
+ * spin, parameter-major; complex handles: interleaved complex).  Enables
+ * kvmc_measure_grad.  No reference counterpart: the reference has no
+ * differentiation. */
 int kvmc_set_orbital_derivs(kvmc_handle* h, int32_t nparams, const double* du_up,
                             const double* du_down);
 
 /* Local energy and log-derivatives O_k(x) = d log det tildeU_up/dtheta_k +
- * d log det tildeU_dn/dtheta_k for every walker.  eloc: nwalkers doubles;
- * o: nwalkers*nparams doubles (walker-major). */
+ * d log det tildeU_dn/dtheta_k for every walker.  Real handles: eloc holds
+ * nwalkers doubles, o nwalkers*nparams doubles (walker-major).  Complex
+ * handles: both are complex, interleaved (re, im): eloc 2*nwalkers doubles
+ * (the local energy <x|H|psi>/<x|psi>, whose real part is getOL's), o
+ * 2*nwalkers*nparams doubles. */
 int kvmc_measure_grad(kvmc_handle* h, double* eloc, double* o);
 
-/* log|det tildeU_up| + log|det tildeU_dn| per walker from the last refresh. */
+/* log|det tildeU_up| + log|det tildeU_dn| per walker from the last refresh
+ * (complex handles too: the modulus). */
 int kvmc_get_logdet(kvmc_handle* h, double* logdet);
 
 /* Copies W_sigma (spin 0 = up, 1 = down) of one walker as FP64, column-major
- * ns x N, with pending delayed updates applied.  The device keeps only the
- * rows of the empty sites; the rows of occupied sites are returned as the
- * unit rows e_label they are by construction. */
+ * ns x N (complex handles: ns x N complex, interleaved), with pending delayed
+ * updates applied.  The device keeps only the rows of the empty sites; the
+ * rows of occupied sites are returned as the unit rows e_label they are by
+ * construction. */
 int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out);
 
 /* Sets / reads ctx.sweeps of every walker (Carlo's sweep counter). */

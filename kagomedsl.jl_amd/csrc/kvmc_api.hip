@@ -151,6 +151,7 @@ struct kvmc_handle {
     int ns = 0, nup = 0, ndn = 0, nb = 0, nmax = 0, nsp = 0;
     int nw = 0;
     int prec = 0, kernel = 0, policy = 0, refresh_k = 0, delay = 1, threads = 0;
+    int cplx = 0;                   // complex orbitals: W in complex float / double
     uint64_t seed = 0;
     size_t esize = 8;
     bool ready = false;
@@ -160,10 +161,17 @@ struct kvmc_handle {
     int2* bonds = nullptr;
     int* nbr_off = nullptr;
     int* nbr = nullptr;
+    // U for the refresh (column-major) and its row-major copy; complex
+    // orbitals: the 2 ns x 2 N real form [U_re, -U_im ; U_im, U_re]
     double* Uup = nullptr;
     double* Udn = nullptr;
     double* UTup = nullptr;   // row-major copies for the refresh gathers
     double* UTdn = nullptr;
+    double* Ucup = nullptr;   // complex orbitals: U itself (interleaved), for Q = dU U^H
+    double* Ucdn = nullptr;
+    int* kRup = nullptr;      // complex orbitals: kappa on the 2 ns sites of the real form
+    int* kRdn = nullptr;
+    double* eloc_im = nullptr;
     int* kup = nullptr;
     int* kdn = nullptr;
     void* W = nullptr;              // empty-site rows per spin (w_stride per walker)
@@ -206,7 +214,8 @@ struct kvmc_handle {
 namespace {
 
 void free_all(kvmc_handle* h) {
-    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->kup, h->kdn, h->W, h->slot, h->A,
+    void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->Ucup, h->Ucdn, h->kRup,
+                    h->kRdn, h->eloc_im, h->kup, h->kdn, h->W, h->slot, h->A,
                     h->Bm, h->npend, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
                     h->tbase, h->rlist, h->rcount, h->failed, h->all_walkers, h->counters,
                     h->logdet, h->eloc, h->drift, h->dstats, h->ws, h->Qup, h->Qdn, h->O};
@@ -252,13 +261,23 @@ SweepArgs sweep_args(kvmc_handle* h) {
 // write_w is 0 only the drift against the current W is measured.
 RefreshArgs refresh_args(const kvmc_handle* h, int write_w, double pivot_tol) {
     RefreshArgs r{};
-    r.ns = h->ns; r.nup = h->nup; r.ndn = h->ndn; r.Uup = h->Uup; r.Udn = h->Udn;
+    const int f = h->cplx ? 2 : 1;   // complex orbitals: the real form's doubled sizes
+    r.ns = f * h->ns; r.nup = f * h->nup; r.ndn = f * h->ndn; r.Uup = h->Uup; r.Udn = h->Udn;
     r.UTup = h->UTup; r.UTdn = h->UTdn;
-    r.kup = h->kup; r.kdn = h->kdn; r.W = h->W; r.slot = h->slot; r.A = h->A; r.Bm = h->Bm;
-    r.npend = h->npend; r.acc_since = h->acc_since; r.delay = h->delay; r.nmax = h->nmax;
+    r.kup = h->cplx ? h->kRup : h->kup; r.kdn = h->cplx ? h->kRdn : h->kdn;
+    r.W = h->W; r.slot = h->slot; r.A = h->A; r.Bm = h->Bm;
+    r.npend = h->npend; r.acc_since = h->acc_since; r.delay = h->delay; r.nmax = f * h->nmax;
+    r.ns_out = h->ns; r.nup_out = h->nup; r.ndn_out = h->ndn; r.nmax_out = h->nmax; r.cplx_out = h->cplx;
     r.ws = h->ws; r.logdet = h->logdet; r.status = h->status; r.failed = h->failed;
     r.write_w = write_w; r.drift = h->drift; r.pivot_tol = pivot_tol;
     return r;
+}
+
+// Complex orbitals: the refresh reads kappa on the 2 ns sites of the real form.
+hipError_t prepare_refresh(kvmc_handle* h, hipStream_t s) {
+    if (!h->cplx) return hipSuccess;
+    hipError_t e = launch_kappa_real_form(h->kup, h->kRup, h->ns, h->nup, h->nw, s);
+    return e != hipSuccess ? e : launch_kappa_real_form(h->kdn, h->kRdn, h->ns, h->ndn, h->nw, s);
 }
 
 int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
@@ -268,6 +287,7 @@ int refresh_list(kvmc_handle* h, const int* d_list, int count, int write_w,
     HIP_TRY(hipMemcpyAsync(h->failed, &kNone, sizeof(int), hipMemcpyHostToDevice, h->stream));
     const int batch_walkers = std::max(1, h->ws_jobs / 2);
     const int fb = (count + 255) / 256;
+    HIP_TRY(prepare_refresh(h, h->stream));
     if (drift_first || !write_w) {
         hipLaunchKernelGGL(k_drift_clear, dim3(fb), dim3(256), 0, h->stream, d_list, count, h->drift);
         HIP_TRY(hipGetLastError());
@@ -341,13 +361,27 @@ int upload_configs(kvmc_handle* h, const int32_t* ku, const int32_t* kd) {
 }
 
 int sweep_threads(const kvmc_handle* h) {
-    const int V = h->prec == 0 ? 2 : 4;
+    const int V = (int)(16 / h->esize);   // W elements per 16-byte vector
     const int nvec = (h->ns % V == 0) ? h->ns / V : h->ns;
     // default: one wavefront when a column fits in it (more walkers per CU
     // hide the proposal latency), else 256 threads
     int t = h->threads > 0 ? h->threads : (nvec <= 64 ? 64 : 256);
     t = std::max(t, (nvec + 63) / 64 * 64);
     return std::min(t, 512);
+}
+
+// Real or complex instantiation of each kernel family.
+hipError_t do_sweep(const kvmc_handle* h, const SweepArgs& a, int nw, int threads, size_t lds, hipStream_t s) {
+    return h->cplx ? launch_sweep_c(h->prec, a, nw, threads, lds, s) : launch_sweep(h->prec, a, nw, threads, lds, s);
+}
+hipError_t do_flush(const kvmc_handle* h, const SweepArgs& a, int nw, hipStream_t s) {
+    return h->cplx ? launch_flush_c(h->prec, a, nw, s) : launch_flush(h->prec, a, nw, s);
+}
+hipError_t do_eloc(const kvmc_handle* h, const ElocArgs& a, hipStream_t s) {
+    return h->cplx ? launch_eloc_c(h->prec, a, s) : launch_eloc(h->prec, a, s);
+}
+hipError_t do_grad(const kvmc_handle* h, const ElocArgs& a, hipStream_t s) {
+    return h->cplx ? launch_grad_c(h->prec, a, s) : launch_grad(h->prec, a, s);
 }
 
 }  // namespace
@@ -384,8 +418,11 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     const int delay = d->delay <= 1 ? 1 : d->delay;
     if (delay > kMaxDelay)
         return fail(KVMC_ERR_INVALID, "delay must be <= " + std::to_string(kMaxDelay));
+    if (d->orbitals != KVMC_ORB_REAL && d->orbitals != KVMC_ORB_COMPLEX)
+        return fail(KVMC_ERR_INVALID, "unknown orbital type");
+    const int cf = d->orbitals == KVMC_ORB_COMPLEX ? 2 : 1;   // the refresh factors a 2N x 2N real form
     if (d->ns > 4096) return fail(KVMC_ERR_INVALID, "ns > 4096 is not supported");
-    if (!refresh_supported(d->ns, d->n_up, d->n_down))
+    if (!refresh_supported(cf * d->ns, cf * d->n_up, cf * d->n_down))
         return fail(KVMC_ERR_INVALID, "lattice too large for the refresh kernel (LDS / rows per thread)");
     for (int b = 0; b < d->nb; ++b) {
         int i = d->bonds[2 * b], j = d->bonds[2 * b + 1];
@@ -399,7 +436,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     h->nw = (int)d->nwalkers;
     h->prec = d->precision; h->kernel = d->kernel; h->policy = d->refresh_policy;
     h->refresh_k = d->refresh_k; h->delay = delay; h->threads = d->threads; h->seed = d->seed;
-    h->esize = h->prec == KVMC_PREC_FP64 ? sizeof(double) : sizeof(float);
+    h->cplx = d->orbitals == KVMC_ORB_COMPLEX ? 1 : 0;
+    h->esize = w_elem_bytes(h->prec, h->cplx);
     (void)hipGetDevice(&h->device);
     int rc = KVMC_OK;
 #define ALLOC(p, n) do { if ((rc = dalloc(&(p), (n))) != KVMC_OK) { free_all(h); delete h; return rc; } } while (0)
@@ -409,10 +447,18 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     ALLOC(h->bonds, (size_t)h->nb);
     ALLOC(h->nbr_off, ns + 1);
     ALLOC(h->nbr, (size_t)2 * h->nb);
-    ALLOC(h->Uup, ns * h->nup);
-    ALLOC(h->Udn, ns * h->ndn);
-    ALLOC(h->UTup, ns * h->nup);
-    ALLOC(h->UTdn, ns * h->ndn);
+    const size_t f2 = (size_t)cf * cf;   // the real form has cf^2 times the entries
+    ALLOC(h->Uup, f2 * ns * h->nup);
+    ALLOC(h->Udn, f2 * ns * h->ndn);
+    ALLOC(h->UTup, f2 * ns * h->nup);
+    ALLOC(h->UTdn, f2 * ns * h->ndn);
+    if (h->cplx) {
+        ALLOC(h->Ucup, 2 * ns * h->nup);
+        ALLOC(h->Ucdn, 2 * ns * h->ndn);
+        ALLOC(h->kRup, 2 * nw * ns);
+        ALLOC(h->kRdn, 2 * nw * ns);
+        ALLOC(h->eloc_im, nw);
+    }
     ALLOC(h->kup, nw * ns);
     ALLOC(h->kdn, nw * ns);
     ALLOC(h->slot, nw * 2 * ns);
@@ -438,12 +484,12 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     // 10000 walkers at 432 sites, so one launch pair covers even a refresh of
     // every walker)
     {
-        const size_t per_job = refresh_ws_doubles_per_job(h->ns, h->nmax) * sizeof(double);
+        const size_t per_job = refresh_ws_doubles_per_job(cf * h->ns, cf * h->nmax) * sizeof(double);
         size_t jobs = std::max<size_t>(2, ((size_t)16 << 30) / per_job);
         jobs = std::min(jobs, 2 * nw);
         h->ws_jobs = (int)(jobs & ~(size_t)1);
         if (h->ws_jobs < 2) h->ws_jobs = 2;
-        ALLOC(h->ws, (size_t)h->ws_jobs * refresh_ws_doubles_per_job(h->ns, h->nmax));
+        ALLOC(h->ws, (size_t)h->ws_jobs * refresh_ws_doubles_per_job(cf * h->ns, cf * h->nmax));
     }
 #undef ALLOC
     // bonds (0-based) and site -> bond adjacency
@@ -459,11 +505,30 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         off[s + 1] = off[s] + (int)adj[s].size();
         lst.insert(lst.end(), adj[s].begin(), adj[s].end());
     }
-    std::vector<double> utu(ns * h->nup), utd(ns * h->ndn);
-    for (size_t s2 = 0; s2 < ns; ++s2) {
-        for (int c = 0; c < h->nup; ++c) utu[s2 * h->nup + c] = d->u_up[(size_t)c * ns + s2];
-        for (int c = 0; c < h->ndn; ++c) utd[s2 * h->ndn + c] = d->u_down[(size_t)c * ns + s2];
-    }
+    // U of the refresh (column-major) and its row-major copy.  Complex
+    // orbitals (interleaved ns x N): the 2 ns x 2 N real form.
+    auto real_form = [&](const double* u, int n, std::vector<double>& cm, std::vector<double>& rm) {
+        const size_t R = (size_t)cf * ns, C = (size_t)cf * n;
+        cm.assign(R * C, 0.0);
+        for (size_t c = 0; c < (size_t)n; ++c)
+            for (size_t s2 = 0; s2 < ns; ++s2) {
+                if (!h->cplx) {
+                    cm[c * R + s2] = u[c * ns + s2];
+                } else {
+                    const double re = u[2 * (c * ns + s2)], im = u[2 * (c * ns + s2) + 1];
+                    cm[c * R + s2] = re;                // [U_re, -U_im ; U_im, U_re]
+                    cm[(c + n) * R + s2] = -im;
+                    cm[c * R + ns + s2] = im;
+                    cm[(c + n) * R + ns + s2] = re;
+                }
+            }
+        rm.resize(R * C);
+        for (size_t r2 = 0; r2 < R; ++r2)
+            for (size_t c = 0; c < C; ++c) rm[r2 * C + c] = cm[c * R + r2];
+    };
+    std::vector<double> ucu, ucd, utu, utd;
+    real_form(d->u_up, h->nup, ucu, utu);
+    real_form(d->u_down, h->ndn, ucd, utd);
     std::vector<int> allw(nw);
     for (size_t w = 0; w < nw; ++w) allw[w] = (int)w;
     auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -472,8 +537,10 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     if (up(h->bonds, hb.data(), hb.size() * sizeof(int2)) != hipSuccess ||
         up(h->nbr_off, off.data(), off.size() * sizeof(int)) != hipSuccess ||
         (lst.size() && up(h->nbr, lst.data(), lst.size() * sizeof(int)) != hipSuccess) ||
-        up(h->Uup, d->u_up, ns * h->nup * sizeof(double)) != hipSuccess ||
-        up(h->Udn, d->u_down, ns * h->ndn * sizeof(double)) != hipSuccess ||
+        up(h->Uup, ucu.data(), ucu.size() * sizeof(double)) != hipSuccess ||
+        up(h->Udn, ucd.data(), ucd.size() * sizeof(double)) != hipSuccess ||
+        (h->cplx && up(h->Ucup, d->u_up, 2 * ns * h->nup * sizeof(double)) != hipSuccess) ||
+        (h->cplx && up(h->Ucdn, d->u_down, 2 * ns * h->ndn * sizeof(double)) != hipSuccess) ||
         up(h->UTup, utu.data(), utu.size() * sizeof(double)) != hipSuccess ||
         up(h->UTdn, utd.data(), utd.size() * sizeof(double)) != hipSuccess ||
         up(h->all_walkers, allw.data(), nw * sizeof(int)) != hipSuccess ||
@@ -615,7 +682,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
     a.tape_u = d_tu; a.tape_bond = d_tb; a.tape_len = nsweeps; a.outcomes = d_out;
     a.walker0 = 0;
     const int threads = sweep_threads(h);
-    const size_t lds = sweep_lds_bytes(h->prec, h->ns, h->nup, h->ndn, h->delay, h->nb);
+    const size_t lds = sweep_lds_bytes(h->esize, h->ns, h->nup, h->ndn, h->delay, h->nb);
     if (lds > 160 * 1024) { cleanup(); return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay"); }
     int64_t launches = 0;
     hipStream_t s = h->stream;
@@ -637,9 +704,10 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
             hipError_t ce = hipSuccess;
             for (int r = 0; r < kGraphRounds && ce == hipSuccess; ++r) {
                 hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->rcount, 0, 1);
-                ce = launch_sweep(h->prec, a, h->nw, threads, lds, s);
+                ce = do_sweep(h, a, h->nw, threads, lds, s);
                 if (r + 1 == kGraphRounds || ce != hipSuccess) break;
                 RefreshArgs ra = refresh_args(h, 1, 0.0);
+                if (ce == hipSuccess) ce = prepare_refresh(h, s);
                 const int batch = std::max(1, h->ws_jobs / 2);
                 for (int b0 = 0; b0 < h->nw && ce == hipSuccess; b0 += batch) {
                     ra.jobs = h->rlist + b0;
@@ -691,7 +759,7 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         int count = 0;
         if (hipMemsetAsync(h->rcount, 0, sizeof(int), s) != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, "hipMemsetAsync failed"); }
         prof_begin(h);
-        hipError_t e = launch_sweep(h->prec, a, h->nw, threads, lds, s);
+        hipError_t e = do_sweep(h, a, h->nw, threads, lds, s);
         if (e != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e)); }
         prof_end(h);
         ++launches;
@@ -733,7 +801,7 @@ static ElocArgs eloc_args(kvmc_handle* h) {
     e.ns = h->ns; e.nup = h->nup; e.ndn = h->ndn; e.nb = h->nb;
     e.bonds = h->bonds; e.kup = h->kup; e.kdn = h->kdn; e.W = h->W; e.slot = h->slot; e.A = h->A; e.Bm = h->Bm;
     e.npend = h->npend; e.delay = h->delay; e.nmax = h->nmax; e.nw = h->nw;
-    e.eloc = h->eloc; e.counters = h->counters;
+    e.eloc = h->eloc; e.eloc_im = h->eloc_im; e.counters = h->counters;
     e.nparams = h->nparams; e.Qup = h->Qup; e.Qdn = h->Qdn; e.O = h->O;
     return e;
 }
@@ -745,7 +813,7 @@ int kvmc_measure(kvmc_handle* h, double* eloc, double* sum, int64_t* n_ap) {
     hipStream_t s = h->stream;
     HIP_TRY(hipMemsetAsync(h->counters + CNT_NAP, 0, sizeof(unsigned long long), s));
     prof_begin(h);
-    HIP_TRY(launch_eloc(h->prec, eloc_args(h), s));
+    HIP_TRY(do_eloc(h, eloc_args(h), s));
     prof_end(h);
     std::vector<double> tmp;
     double* dst = eloc;
@@ -776,18 +844,23 @@ int kvmc_set_orbital_derivs(kvmc_handle* h, int32_t nparams, const double* du_up
     h->nparams = 0;
     if (nparams == 0) return KVMC_OK;
     const size_t ns = h->ns;
+    const size_t cw = h->cplx ? 2 : 1;   // doubles per value (complex: interleaved)
     int rc;
     // built in scope-owned buffers; the handle takes them only on success
     DevBuf<double> qu, qd, o, dU;
-    if ((rc = qu.alloc((size_t)nparams * ns * ns)) || (rc = qd.alloc((size_t)nparams * ns * ns)) ||
-        (rc = o.alloc((size_t)nparams * h->nw)) || (rc = dU.alloc((size_t)nparams * ns * h->nmax)))
+    if ((rc = qu.alloc(cw * nparams * ns * ns)) || (rc = qd.alloc(cw * nparams * ns * ns)) ||
+        (rc = o.alloc(cw * nparams * h->nw)) || (rc = dU.alloc(cw * nparams * ns * h->nmax)))
         return rc;
     hipStream_t s = h->stream;
-    HIP_TRY(hipMemcpyAsync(dU.p, du_up, (size_t)nparams * ns * h->nup * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_q_build(dU.p, h->Uup, h->ns, h->nup, nparams, qu.p, s));
+    auto q_build = [&](const double* U, int n, double* Q) {
+        return h->cplx ? launch_q_build_c(dU.p, U, h->ns, n, nparams, Q, s)
+                       : launch_q_build(dU.p, U, h->ns, n, nparams, Q, s);
+    };
+    HIP_TRY(hipMemcpyAsync(dU.p, du_up, cw * nparams * ns * h->nup * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(q_build(h->cplx ? h->Ucup : h->Uup, h->nup, qu.p));
     HIP_TRY(hipStreamSynchronize(s));
-    HIP_TRY(hipMemcpyAsync(dU.p, du_dn, (size_t)nparams * ns * h->ndn * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_q_build(dU.p, h->Udn, h->ns, h->ndn, nparams, qd.p, s));
+    HIP_TRY(hipMemcpyAsync(dU.p, du_dn, cw * nparams * ns * h->ndn * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(q_build(h->cplx ? h->Ucdn : h->Udn, h->ndn, qd.p));
     HIP_TRY(hipStreamSynchronize(s));
     h->Qup = qu.release();
     h->Qdn = qd.release();
@@ -803,12 +876,18 @@ int kvmc_measure_grad(kvmc_handle* h, double* eloc, double* o) {
     if (h->nparams <= 0) return fail(KVMC_ERR_INVALID, "orbital derivatives not set");
     hipStream_t s = h->stream;
     SweepArgs a = sweep_args(h);
-    HIP_TRY(launch_flush(h->prec, a, h->nw, s));
-    HIP_TRY(launch_eloc(h->prec, eloc_args(h), s));
-    HIP_TRY(launch_grad(h->prec, eloc_args(h), s));
-    if (eloc) HIP_TRY(hipMemcpyAsync(eloc, h->eloc, sizeof(double) * h->nw, hipMemcpyDeviceToHost, s));
-    if (o) HIP_TRY(hipMemcpyAsync(o, h->O, sizeof(double) * h->nw * h->nparams, hipMemcpyDeviceToHost, s));
+    HIP_TRY(do_flush(h, a, h->nw, s));
+    HIP_TRY(do_eloc(h, eloc_args(h), s));
+    HIP_TRY(do_grad(h, eloc_args(h), s));
+    const size_t cw = h->cplx ? 2 : 1;
+    std::vector<double> re, im;
+    if (eloc && h->cplx) { re.resize(h->nw); im.resize(h->nw); }
+    if (eloc) HIP_TRY(hipMemcpyAsync(h->cplx ? re.data() : eloc, h->eloc, sizeof(double) * h->nw, hipMemcpyDeviceToHost, s));
+    if (eloc && h->cplx) HIP_TRY(hipMemcpyAsync(im.data(), h->eloc_im, sizeof(double) * h->nw, hipMemcpyDeviceToHost, s));
+    if (o) HIP_TRY(hipMemcpyAsync(o, h->O, cw * sizeof(double) * h->nw * h->nparams, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (eloc && h->cplx)
+        for (int w = 0; w < h->nw; ++w) { eloc[2 * w] = re[w]; eloc[2 * w + 1] = im[w]; }
     return KVMC_OK;
 }
 
@@ -818,7 +897,9 @@ int kvmc_get_logdet(kvmc_handle* h, double* logdet) {
     std::vector<double> two((size_t)2 * h->nw);
     HIP_TRY(hipMemcpyAsync(two.data(), h->logdet, two.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int w = 0; w < h->nw; ++w) logdet[w] = two[2 * w] + two[2 * w + 1];
+    // complex orbitals: the real form's determinant is |det tildeU|^2
+    const double f = h->cplx ? 0.5 : 1.0;
+    for (int w = 0; w < h->nw; ++w) logdet[w] = f * (two[2 * w] + two[2 * w + 1]);
     return KVMC_OK;
 }
 
@@ -829,19 +910,20 @@ int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out) {
     hipStream_t s = h->stream;
     SweepArgs a = sweep_args(h);
     a.walker0 = (int)walker;
-    HIP_TRY(launch_flush(h->prec, a, 1, s));
+    HIP_TRY(do_flush(h, a, 1, s));
     // W keeps the empty-site rows; the occupied rows are the unit rows e_label
     const size_t ns = h->ns, n = spin ? h->ndn : h->nup, ld = w_ld(h->ns, (int)n);
-    const size_t off = (size_t)walker * w_stride(h->ns, h->nup, h->ndn) + (spin ? w_off_dn(h->ns, h->nup) : 0);
-    std::vector<double> rows(ld * n);
+    const size_t cw = h->cplx ? 2 : 1;   // complex: (re, im) pairs
+    const size_t off = cw * ((size_t)walker * w_stride(h->ns, h->nup, h->ndn) + (spin ? w_off_dn(h->ns, h->nup) : 0));
+    std::vector<double> rows(cw * ld * n);
     std::vector<int> sl(ns), kap(ns);
     if (h->prec == KVMC_PREC_FP64) {
-        HIP_TRY(hipMemcpyAsync(rows.data(), (double*)h->W + off, ld * n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(rows.data(), (double*)h->W + off, rows.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     } else {
-        std::vector<float> tmp(ld * n);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)h->W + off, ld * n * sizeof(float), hipMemcpyDeviceToHost, s));
+        std::vector<float> tmp(rows.size());
+        HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)h->W + off, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        for (size_t q = 0; q < ld * n; ++q) rows[q] = tmp[q];
+        for (size_t q = 0; q < rows.size(); ++q) rows[q] = tmp[q];
     }
     HIP_TRY(hipMemcpyAsync(sl.data(), h->slot + ((size_t)walker * 2 + spin) * ns, ns * sizeof(int),
                            hipMemcpyDeviceToHost, s));
@@ -850,8 +932,10 @@ int kvmc_get_w(kvmc_handle* h, int64_t walker, int32_t spin, double* out) {
     HIP_TRY(hipStreamSynchronize(s));
     for (size_t l = 0; l < n; ++l)
         for (size_t r = 0; r < ns; ++r)
-            out[l * ns + r] = (sl[r] >= 0 && (size_t)sl[r] < ld) ? rows[l * ld + sl[r]]
-                                                                 : (kap[r] == (int)l + 1 ? 1.0 : 0.0);
+            for (size_t p = 0; p < cw; ++p)
+                out[cw * (l * ns + r) + p] = (sl[r] >= 0 && (size_t)sl[r] < ld)
+                                                 ? rows[cw * (l * ld + sl[r]) + p]
+                                                 : (p == 0 && kap[r] == (int)l + 1 ? 1.0 : 0.0);
     return KVMC_OK;
 }
 

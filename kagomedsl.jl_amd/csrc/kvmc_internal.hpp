@@ -19,6 +19,40 @@ enum : int { CNT_MOVES = 0, CNT_ACCEPTED = 1, CNT_FLUSHES = 2, CNT_SWEEPS = 3, C
 // Largest number of pending rank-1 updates kept per spin before a flush.
 constexpr int kMaxDelay = 16;
 
+// Complex W element (complex orbitals: flux phases or a magnetic field make
+// the spinon Hamiltonian complex, src/Hamiltonian.jl:201, 325, 382-393).
+// Stored interleaved (re, im) like Julia's ComplexF64 / ComplexF32.
+template <typename R>
+struct __align__(2 * sizeof(R)) cplx {
+    R re, im;
+    cplx() = default;
+    __host__ __device__ constexpr cplx(R r) : re(r), im(R(0)) {}
+    __host__ __device__ constexpr cplx(R r, R i) : re(r), im(i) {}
+    template <typename S>
+    __host__ __device__ explicit constexpr cplx(const cplx<S>& o) : re((R)o.re), im((R)o.im) {}
+};
+template <typename R> __host__ __device__ inline cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator-(cplx<R> a) { return {-a.re, -a.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R> __host__ __device__ inline cplx<R> operator*(R a, cplx<R> b) { return {a * b.re, a * b.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator/(cplx<R> a, cplx<R> b) {
+    const R d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+template <typename R> __host__ __device__ inline cplx<R>& operator+=(cplx<R>& a, cplx<R> b) { a = a + b; return a; }
+template <typename R> __host__ __device__ inline cplx<R>& operator-=(cplx<R>& a, cplx<R> b) { a = a - b; return a; }
+typedef cplx<float> cf32;
+typedef cplx<double> cf64;
+
+template <typename T> struct is_cplx { static constexpr bool value = false; };
+template <typename R> struct is_cplx<cplx<R>> { static constexpr bool value = true; };
+// Real scalar of a storage type (float for cf32, double for cf64).
+template <typename T> struct real_of { using type = T; };
+template <typename R> struct real_of<cplx<R>> { using type = R; };
+
 // W of one spin keeps only the rows of the empty sites ("slots"): the row of
 // an occupied site is the unit row e_label by construction (W = U tildeU^-1,
 // W[R_l, :] = e_l).  Per walker: W_up (ldw_up x nup) then W_dn (ldw_dn x
@@ -128,6 +162,15 @@ struct RefreshArgs {
     int write_w;                    // 1: replace W; 0: measure drift only
     double pivot_tol;               // reject when min|pivot| < pivot_tol * max|pivot|
     double* drift;                  // [nw][2] per walker: max |W_new - W_cur|, max |W_new| (ordered bits)
+    // Storage shape of W, the slot map and the pending terms.  Real orbitals:
+    // the same as the factorised problem.  Complex orbitals (cplx_out = 1):
+    // the refresh factors the real 2N x 2N form [[Re, -Im], [Im, Re]] of
+    // tildeU with rows [U_re, -U_im ; U_im, U_re] of the 2 ns x 2 N real
+    // form of U and kappa extended to 2 ns sites (site ns + s holds label
+    // N + l when s holds l), so ns = 2 ns_out and N = 2 N_out; the rows of
+    // the first ns_out sites give W = X_re + i X_im with X = [X_re, -X_im].
+    int ns_out, nup_out, ndn_out, nmax_out;
+    int cplx_out;
 };
 
 struct ElocArgs {
@@ -143,28 +186,44 @@ struct ElocArgs {
     int delay, nmax;
     int nw;
     double* eloc;                   // [nw]
+    double* eloc_im;                // [nw] imaginary part (complex W only; may be null)
     unsigned long long* counters;   // CNT_NAP accumulates antiparallel counts
-    // log-derivatives: Q_k = dU_k U^T (ns x ns row-major per parameter and spin)
+    // log-derivatives: Q_k = dU_k U^H (ns x ns row-major per parameter and
+    // spin; complex interleaved for complex W)
     int nparams;
     const double* Qup;              // [nparams][ns][ns]
     const double* Qdn;
-    double* O;                      // [nw][nparams]
+    double* O;                      // [nw][nparams] (complex: [nw][nparams][2])
 };
 
-// Launchers (defined in kvmc_device.hip).  `prec` 0 = FP64, 1 = FP32.
+// Storage type of W: `prec` 0 = FP64, 1 = FP32; `cplx` 0 = real orbitals,
+// 1 = complex orbitals.
+inline size_t w_elem_bytes(int prec, int cplx) { return (prec == 0 ? 8 : 4) * (cplx ? 2 : 1); }
+
+// Launchers.  Real W: kvmc_device.hip; complex W: kvmc_complex.hip.
 hipError_t launch_sweep(int prec, const SweepArgs& a, int nwalkers, int threads, size_t lds,
                         hipStream_t s);
-size_t sweep_lds_bytes(int prec, int ns, int nup, int ndn, int delay, int nb);
+hipError_t launch_sweep_c(int prec, const SweepArgs& a, int nwalkers, int threads, size_t lds,
+                          hipStream_t s);
+size_t sweep_lds_bytes(size_t esize, int ns, int nup, int ndn, int delay, int nb);
 // Both W of a walker at most this large: the sweep kernel keeps them in LDS.
 constexpr size_t kResidentMaxBytes = 64 * 1024;
-bool sweep_resident(int prec, int ns, int nup, int ndn);
+bool sweep_resident(size_t esize, int ns, int nup, int ndn);
 hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s);
 size_t refresh_ws_doubles_per_job(int ns, int nmax);
 bool refresh_supported(int ns, int nup, int ndn);
 hipError_t launch_eloc(int prec, const ElocArgs& a, hipStream_t s);
+hipError_t launch_eloc_c(int prec, const ElocArgs& a, hipStream_t s);
 hipError_t launch_grad(int prec, const ElocArgs& a, hipStream_t s);
+hipError_t launch_grad_c(int prec, const ElocArgs& a, hipStream_t s);
 hipError_t launch_flush(int prec, const SweepArgs& a, int nwalkers, hipStream_t s);
+hipError_t launch_flush_c(int prec, const SweepArgs& a, int nwalkers, hipStream_t s);
 hipError_t launch_q_build(const double* dU, const double* U, int ns, int n, int nparams,
                           double* Q, hipStream_t s);
+hipError_t launch_q_build_c(const double* dU, const double* U, int ns, int n, int nparams,
+                            double* Q, hipStream_t s);
+// kappa of every walker extended to the real form of a complex refresh:
+// out[w][s] = kap[w][s], out[w][ns + s] = kap[w][s] ? kap[w][s] + n : 0.
+hipError_t launch_kappa_real_form(const int* kap, int* out, int ns, int n, int nw, hipStream_t s);
 
 }  // namespace kvmc

@@ -600,11 +600,13 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     // ---- 3. W from X, label column by label column ------------------------
     // Empty site s takes row rowof[s] - N (site order); occupied rows are unit
     // rows and are not stored.  Drift mode reads the current rows through the
-    // current slot map.
-    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn) +
-                         (spin ? w_off_dn(ns, a.nup) : 0);
-    const int ld = w_ld(ns, N);
-    int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * ns;
+    // current slot map.  Complex orbitals: only the first ns_out sites are
+    // stored; label l < N_out gives Re W[., l], label N_out + l gives -Im.
+    const int nso = a.ns_out, NO = spin ? a.ndn_out : a.nup_out, cw = a.cplx_out ? 2 : 1;
+    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) +
+                         ((size_t)w * w_stride(nso, a.nup_out, a.ndn_out) + (spin ? w_off_dn(nso, a.nup_out) : 0)) * cw;
+    const int ld = w_ld(nso, NO);
+    int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * nso;
     // Blocks of SB consecutive sites: the solved rows of the block's empty
     // sites are staged in LDS (contiguous row reads, odd row stride against
     // bank conflicts), then every label column of W gets the block's entries
@@ -612,16 +614,16 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     constexpr int SB = 16;
     const int xs = (nc16 | 1);                       // LDS row stride (odd)
     double* xb = stage;                              // [SB][xs]
-    const int nsa = w_nsa(ns, a.nup, a.ndn);
+    const int nsa = w_nsa(nso, a.nup_out, a.ndn_out);
     const int cpend = a.write_w ? 0 : a.npend[w];
     const int dp = pend_ld(a.delay);
-    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp;
-    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * nmax * dp;
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp * cw;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.nmax_out * dp * cw;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
-    for (int sb = 0; sb < ns; sb += SB) {
+    for (int sb = 0; sb < nso; sb += SB) {
         for (int i = wave; i < SB; i += NWAVE) {
             const int s = sb + i;
-            if (s < ns && kap[s] == 0) {
+            if (s < nso && kap[s] == 0) {
                 const double* __restrict__ src = M + (size_t)rowof[s] * ldm;
                 for (int c = lane; c < N; c += 64) xb[i * xs + c] = src[c];
             }
@@ -629,15 +631,24 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         __syncthreads();
         for (int idx = t; idx < N * SB; idx += RT) {
             const int l = idx / SB, i = idx % SB, s = sb + i;
-            if (s >= ns || kap[s] != 0) continue;
-            const double v = xb[i * xs + pinv[l]];
+            if (s >= nso || kap[s] != 0) continue;
+            const int part = l >= NO ? 1 : 0, lo = l - part * NO;
+            const double v = part ? -xb[i * xs + pinv[l]] : xb[i * xs + pinv[l]];
             const int row = a.write_w ? rowof[s] - N : max(sl[s], 0);
-            T* __restrict__ dst = Wo + (size_t)l * ld + row;
+            const size_t at = (size_t)lo * ld + row;
             if (a.write_w) {
-                *dst = (T)v;
+                Wo[at * cw + part] = (T)v;
             } else {
-                double cur = (double)*dst;
-                for (int c = 0; c < cpend; ++c) cur += (double)gA[(size_t)row * dp + c] * (double)gB[(size_t)l * dp + c];
+                double cur = (double)Wo[at * cw + part];
+                for (int c = 0; c < cpend; ++c) {
+                    const size_t ia = ((size_t)row * dp + c) * cw, ib = ((size_t)lo * dp + c) * cw;
+                    if (cw == 1) {
+                        cur += (double)gA[ia] * (double)gB[ib];
+                    } else {   // component `part` of the complex product
+                        const double ar = gA[ia], ai = gA[ia + 1], br = gB[ib], bi = gB[ib + 1];
+                        cur += part ? ar * bi + ai * br : ar * br - ai * bi;
+                    }
+                }
                 dmax = fmax(dmax, fabs(v - cur));
                 vmax = fmax(vmax, fabs(v));
             }
@@ -645,7 +656,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         __syncthreads();
     }
     if (a.write_w) {
-        for (int s = t; s < ns; s += RT) sl[s] = kap[s] == 0 ? rowof[s] - N : -1;
+        for (int s = t; s < nso; s += RT) sl[s] = kap[s] == 0 ? rowof[s] - N : -1;
     } else {
         atomic_max_nonneg2(&a.drift[(size_t)w * 2], dmax);
         atomic_max_nonneg2(&a.drift[(size_t)w * 2 + 1], vmax);
@@ -722,7 +733,10 @@ k_refresh_solve(RefreshArgs a, int groups) {
     const int w = a.jobs[job >> 1], spin = job & 1;
     const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
     const int nE = ns - N;
-    if (grp > 0 && 64 * grp >= nE) return;
+    // rows stored: complex orbitals keep the first ns_out sites only
+    const int nso = a.ns_out, NO = spin ? a.ndn_out : a.nup_out, cw = a.cplx_out ? 2 : 1;
+    const int nEo = nso - NO;
+    if (grp > 0 && 64 * grp >= nEo) return;
     const int nt = (N + 15) / 16;
     const double* __restrict__ M = a.ws + (size_t)job * refresh_ws_job(ns, nmax);
     const int ldm = refresh_ldm(nmax);
@@ -759,8 +773,8 @@ k_refresh_solve(RefreshArgs a, int groups) {
     }
 
     const int e = 64 * grp + 16 * wave + (lane & 15);
-    const int site = e < nE ? esite[e] : -1;
-    const bool active = 64 * grp + 16 * wave < nE;   // wavefront holds at least one site
+    const int site = e < nEo ? esite[e] : -1;
+    const bool active = 64 * grp + 16 * wave < nEo;   // wavefront holds at least one site
     const int kr = lane >> 4, kc = lane & 15;
 
     // The tiles hold the accumulators negated: T_K = -acc_K.  The diagonal
@@ -854,15 +868,15 @@ k_refresh_solve(RefreshArgs a, int groups) {
     // Write mode: the empty sites take rows 0..nE-1 in site order and the
     // map is rewritten for this workgroup's span of sites.  Drift mode reads
     // the current rows through the current map.
-    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn) +
-                         (spin ? w_off_dn(ns, a.nup) : 0);
-    const int ld = w_ld(ns, N);
-    int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * ns;
-    const int nsa = w_nsa(ns, a.nup, a.ndn);
+    T* __restrict__ Wo = reinterpret_cast<T*>(a.W) +
+                         ((size_t)w * w_stride(nso, a.nup_out, a.ndn_out) + (spin ? w_off_dn(nso, a.nup_out) : 0)) * cw;
+    const int ld = w_ld(nso, NO);
+    int* __restrict__ sl = a.slot + ((size_t)w * 2 + spin) * nso;
+    const int nsa = w_nsa(nso, a.nup_out, a.ndn_out);
     const int cpend = a.write_w ? 0 : a.npend[w];
     const int dp = pend_ld(a.delay);
-    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp;
-    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * nmax * dp;
+    const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp * cw;
+    const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.nmax_out * dp * cw;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
     if (site >= 0) {
         const int row = a.write_w ? e : max(sl[site], 0);
@@ -872,14 +886,24 @@ k_refresh_solve(RefreshArgs a, int groups) {
             for (int g = 0; g < 4; ++g) {
                 const int c = 16 * J + kr + 4 * g;
                 if (c < N) {
+                    // complex orbitals: label l < N_out is Re W[., l], N_out + l is -Im
                     const int l = lab[c];
-                    const double v = tile[J][g];
-                    T* __restrict__ dst = Wo + (size_t)l * ld + row;
+                    const int part = l >= NO ? 1 : 0, lo = l - part * NO;
+                    const double v = part ? -tile[J][g] : tile[J][g];
+                    const size_t at = (size_t)lo * ld + row;
                     if (a.write_w) {
-                        *dst = (T)v;
+                        Wo[at * cw + part] = (T)v;
                     } else {
-                        double cur = (double)*dst;
-                        for (int q = 0; q < cpend; ++q) cur += (double)gA[(size_t)row * dp + q] * (double)gB[(size_t)l * dp + q];
+                        double cur = (double)Wo[at * cw + part];
+                        for (int q = 0; q < cpend; ++q) {
+                            const size_t ia = ((size_t)row * dp + q) * cw, ib = ((size_t)lo * dp + q) * cw;
+                            if (cw == 1) {
+                                cur += (double)gA[ia] * (double)gB[ib];
+                            } else {
+                                const double ar = gA[ia], ai = gA[ia + 1], br = gB[ib], bi = gB[ib + 1];
+                                cur += part ? ar * bi + ai * br : ar * br - ai * bi;
+                            }
+                        }
                         dmax = fmax(dmax, fabs(v - cur));
                         vmax = fmax(vmax, fabs(v));
                     }
@@ -889,10 +913,10 @@ k_refresh_solve(RefreshArgs a, int groups) {
     }
     if (a.write_w) {
         const int s_lo = grp == 0 ? 0 : esite[64 * grp];
-        const int s_hi = 64 * (grp + 1) < nE ? esite[64 * (grp + 1)] : ns;
+        const int s_hi = 64 * (grp + 1) < nEo ? esite[64 * (grp + 1)] : nso;
         for (int s = s_lo + t; s < s_hi; s += RT)
             if (kap[s] != 0) sl[s] = -1;
-        if (t < 64 && 64 * grp + t < nE) sl[esite[64 * grp + t]] = 64 * grp + t;
+        if (t < 64 && 64 * grp + t < nEo) sl[esite[64 * grp + t]] = 64 * grp + t;
     } else {
         atomic_max_nonneg2(&a.drift[(size_t)w * 2], dmax);
         atomic_max_nonneg2(&a.drift[(size_t)w * 2 + 1], vmax);
@@ -935,8 +959,9 @@ static hipError_t launch_lu(const RefreshArgs& a, size_t lds, hipStream_t s) {
 
 template <typename T, int NT>
 static hipError_t launch_solve(const RefreshArgs& a, hipStream_t s) {
-    const int nmin = a.nup < a.ndn ? a.nup : a.ndn;
-    const int groups = (a.ns - nmin + 63) / 64 > 1 ? (a.ns - nmin + 63) / 64 : 1;
+    // workgroups of 64 stored empty-site rows per job
+    const int nmin = a.nup_out < a.ndn_out ? a.nup_out : a.ndn_out;
+    const int groups = (a.ns_out - nmin + 63) / 64 > 1 ? (a.ns_out - nmin + 63) / 64 : 1;
     const size_t lds = solve_lds_bytes(a.ns, NT);
     if (solve_static_lds(NT) + lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_refresh_solve<T, NT>),

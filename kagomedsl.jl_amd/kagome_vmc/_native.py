@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libkagome_vmc.so")
 
 KVMC_OK, KVMC_ERR_INVALID, KVMC_ERR_SINGULAR, KVMC_ERR_HIP, KVMC_ERR_NOMEM, KVMC_ERR_NOT_READY = range(6)
 PREC_FP64, PREC_FP32 = 0, 1
+ORB_REAL, ORB_COMPLEX = 0, 1
 KERNEL_REFERENCE, KERNEL_UNBIASED = 0, 1
 REFRESH_REFERENCE, REFRESH_EVERY_K, REFRESH_AT_END = 0, 1, 2
 OUT_PRE_REJECT, OUT_NO_MOVE, OUT_REJECT, OUT_ACCEPT = 0, 1, 2, 3
@@ -36,6 +37,7 @@ class KvmcDesc(C.Structure):
         ("u_down", C.POINTER(C.c_double)), ("nwalkers", C.c_int64), ("precision", C.c_int32),
         ("kernel", C.c_int32), ("refresh_policy", C.c_int32), ("refresh_k", C.c_int32),
         ("delay", C.c_int32), ("threads", C.c_int32), ("seed", C.c_uint64),
+        ("orbitals", C.c_int32),
     ]
 
 
@@ -131,14 +133,22 @@ class DeviceSampler:
 
     def __init__(self, bonds, u_up, u_down, nwalkers, *, precision=PREC_FP64,
                  kernel=KERNEL_REFERENCE, refresh_policy=REFRESH_REFERENCE, refresh_k=0, delay=1,
-                 threads=0, seed=0):
+                 threads=0, seed=0, orbitals=None):
         L = load()
         self.bonds = np.ascontiguousarray(np.asarray(bonds, dtype=np.int32).reshape(-1, 2))
-        if np.iscomplexobj(u_up) or np.iscomplexobj(u_down):
-            if np.any(np.imag(u_up) != 0) or np.any(np.imag(u_down) != 0):
-                raise ValueError("complex orbitals are not supported by the device sampler")
-        self.u_up = np.asfortranarray(np.real(u_up), dtype=np.float64)
-        self.u_down = np.asfortranarray(np.real(u_down), dtype=np.float64)
+        # complex orbitals (flux phases, magnetic field) give a complex W;
+        # orbitals with zero imaginary part run as real unless `orbitals`
+        # asks for ORB_COMPLEX (e.g. real U with complex derivatives)
+        if orbitals is None:
+            self.complex = bool((np.iscomplexobj(u_up) and np.any(np.imag(u_up) != 0)) or
+                                (np.iscomplexobj(u_down) and np.any(np.imag(u_down) != 0)))
+        else:
+            self.complex = orbitals == ORB_COMPLEX
+            if not self.complex and (np.any(np.imag(u_up) != 0) or np.any(np.imag(u_down) != 0)):
+                raise ValueError("complex orbitals need orbitals=ORB_COMPLEX")
+        dt = np.complex128 if self.complex else np.float64
+        self.u_up = np.asfortranarray(u_up if self.complex else np.real(u_up), dtype=dt)
+        self.u_down = np.asfortranarray(u_down if self.complex else np.real(u_down), dtype=dt)
         self.ns, self.n_up = self.u_up.shape
         self.n_down = self.u_down.shape[1]
         self.nb = len(self.bonds)
@@ -152,6 +162,7 @@ class DeviceSampler:
         d.nwalkers = self.nwalkers
         d.precision, d.kernel, d.refresh_policy = precision, kernel, refresh_policy
         d.refresh_k, d.delay, d.threads, d.seed = refresh_k, delay, threads, seed
+        d.orbitals = ORB_COMPLEX if self.complex else ORB_REAL
         h = C.c_void_p()
         _check(L.kvmc_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -216,8 +227,13 @@ class DeviceSampler:
         return e, nap.value
 
     def set_orbital_derivs(self, du_up, du_down):
-        du_up = np.asarray(du_up, dtype=np.float64)
-        du_down = np.asarray(du_down, dtype=np.float64)
+        dt = np.complex128 if self.complex else np.float64
+        if not self.complex and (np.iscomplexobj(du_up) or np.iscomplexobj(du_down)):
+            if np.any(np.imag(du_up) != 0) or np.any(np.imag(du_down) != 0):
+                raise ValueError("complex orbital derivatives need complex orbitals")
+            du_up, du_down = np.real(du_up), np.real(du_down)
+        du_up = np.asarray(du_up, dtype=dt)
+        du_down = np.asarray(du_down, dtype=dt)
         P = du_up.shape[0]
         a = np.ascontiguousarray(np.stack([np.asfortranarray(du_up[k]).ravel(order="F")
                                            for k in range(P)]))
@@ -227,8 +243,12 @@ class DeviceSampler:
         self.nparams = P
 
     def measure_grad(self):
-        e = np.zeros(self.nwalkers, dtype=np.float64)
-        o = np.zeros((self.nwalkers, self.nparams), dtype=np.float64)
+        """Local energies and log-derivatives O_k per walker (complex arrays
+        for complex orbitals: E_L = <x|H|psi>/<x|psi>, whose real part is
+        the reference's getOL)."""
+        dt = np.complex128 if self.complex else np.float64
+        e = np.zeros(self.nwalkers, dtype=dt)
+        o = np.zeros((self.nwalkers, self.nparams), dtype=dt)
         _check(load().kvmc_measure_grad(self._h, _ptr(e), _ptr(o)))
         return e, o
 
@@ -239,7 +259,7 @@ class DeviceSampler:
 
     def get_w(self, walker, spin):
         n = self.n_up if spin == 0 else self.n_down
-        out = np.zeros(self.ns * n, dtype=np.float64)
+        out = np.zeros(self.ns * n, dtype=np.complex128 if self.complex else np.float64)
         _check(load().kvmc_get_w(self._h, int(walker), int(spin), _ptr(out)))
         return out.reshape((self.ns, n), order="F")
 

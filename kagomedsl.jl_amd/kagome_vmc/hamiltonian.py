@@ -29,6 +29,45 @@ pi_link_inter = {
 }
 
 
+def link_orientation(lat: DoubleKagome, l1: int, l2: int, dx: int = 0, dy: int = 0) -> int:
+    """+1 when the hop from label l1 (cell 0) to label l2 in the cell
+    dx a1 + dy a2 runs counterclockwise around the triangle that holds the
+    bond, -1 otherwise.  Every kagome bond belongs to exactly one triangle;
+    its third vertex is the site at distance t from both ends."""
+    r1 = lat.r[l1 - 1]
+    r2 = dx * lat.a1 + dy * lat.a2 + lat.r[l2 - 1]
+    for l in range(6):
+        for cx in range(-2, 3):
+            for cy in range(-2, 3):
+                rk = cx * lat.a1 + cy * lat.a2 + lat.r[l]
+                if (abs(np.linalg.norm(rk - r1) - lat.t) < 1e-9 and
+                        abs(np.linalg.norm(rk - r2) - lat.t) < 1e-9):
+                    d1, d2 = r2 - r1, rk - r1
+                    return 1 if d1[0] * d2[1] - d1[1] * d2[0] > 0 else -1
+    raise ValueError(f"link ({l1}, {l2}, {dx}, {dy}) is not a nearest-neighbour bond")
+
+
+def flux_link_tables(lat: DoubleKagome, phi: float, link_in=None, link_inter=None, dphi: bool = False):
+    """Link tables of the flux family: each amplitude of the pi-flux tables
+    (src/Hamiltonian.jl:219-261) times exp(i phi o / 3), o the link's
+    orientation in its triangle.  Every triangle then carries flux phi and
+    every hexagon pi - 2 phi (each hexagon edge is traversed against its
+    triangle), and phi = 0 is the pi-flux ansatz.  Complex amplitudes enter
+    Hmat exactly as in the reference, whose tables hold numbers multiplying a
+    ComplexF64 matrix (custom tables: scripts/zero_flux.jl:15-42).  With
+    dphi, the tables of the derivative d/dphi (Hmat is linear in them)."""
+    link_in = pi_link_in if link_in is None else link_in
+    link_inter = pi_link_inter if link_inter is None else link_inter
+
+    def amp(v, o):
+        a = v * np.exp(1j * phi * o / 3.0)
+        return a * (1j * o / 3.0) if dphi else a
+
+    tin = {k: amp(v, link_orientation(lat, k[0], k[1])) for k, v in link_in.items()}
+    tinter = {k: amp(v, link_orientation(lat, *k)) for k, v in link_inter.items()}
+    return tin, tinter
+
+
 def _peierls(B, r1, r2):
     """exp(i B/2 (x1 + x2)(y2 - y1)) (src/Hamiltonian.jl:201-202,325-326)."""
     return np.exp(1j * (B / 2.0) * (r1[..., 0] + r2[..., 0]) * (r2[..., 1] - r1[..., 1]))

@@ -184,11 +184,12 @@ int kvo_boundary_shifts(const kvo_lattice* lat, int s1, int s2, int* dxs, int* d
 
 /* Hmat: src/Hamiltonian.jl:308-354 with apply_boundary_conditions! (:177-207).
  * link_in: n_in entries (l1, l2, value); link_inter: n_inter entries
- * (l1, l2, dx, dy, value).  H is ns x ns, column-major, H = -(T + T').
+ * (l1, l2, dx, dy, value); values are complex (the reference's Dict values
+ * multiply a ComplexF64 matrix).  H is ns x ns, column-major, H = -(T + T').
  * Returns 0, or 3 if T is not upper triangular (the reference's error). */
 int kvo_hmat(const kvo_lattice* lat, int n_in, const int* in_l1, const int* in_l2,
-             const double* in_val, int n_inter, const int* it_l1, const int* it_l2,
-             const int* it_dx, const int* it_dy, const double* it_val, double B, cplx* H) {
+             const cplx* in_val, int n_inter, const int* it_l1, const int* it_l2,
+             const int* it_dx, const int* it_dy, const cplx* it_val, double B, cplx* H) {
     int n1 = lat->n1, n2 = lat->n2;
     int ns = n1 * n2 * 3;
     int ncell = n1 * n2 / 2;
@@ -413,10 +414,11 @@ int kvo_z(int nb, const int* bonds, const int* kup, const int* kdn) {
 
 /* getOL: src/Hamiltonian.jl:762-778 with getxprime (:711-720),
  * spinInteraction! (:636-672), SzInteraction! (:593-604) and Sz (:531-565).
- * Returns real(OL).  *err = 1 if a bond touches an empty or doubly occupied
- * site (Sz's ArgumentError). */
-double kvo_getol(int ns, int nup, int ndn, int nb, const int* bonds, const int* kup,
-                 const int* kdn, const cplx* Wup, const cplx* Wdn, int* err) {
+ * Returns the complex sum of the diagonal and exchange terms; the reference's
+ * OL is its real part.  *err = 1 if a bond touches an empty or doubly
+ * occupied site (Sz's ArgumentError). */
+static cplx getol_sum(int ns, int nb, const int* bonds, const int* kup, const int* kdn, const cplx* Wup,
+                      const cplx* Wdn, int* err) {
     double diag = 0.0;
     cplx off = 0.0;
     *err = 0;
@@ -432,8 +434,22 @@ double kvo_getol(int ns, int nup, int ndn, int nb, const int* bonds, const int* 
         double szi = ui ? 0.5 : -0.5, szj = uj ? 0.5 : -0.5;
         diag += szi * szj;
     }
+    return diag + off;
+}
+
+double kvo_getol(int ns, int nup, int ndn, int nb, const int* bonds, const int* kup,
+                 const int* kdn, const cplx* Wup, const cplx* Wdn, int* err) {
     (void)nup; (void)ndn;
-    return creal(diag + off);
+    return creal(getol_sum(ns, nb, bonds, kup, kdn, Wup, Wdn, err));
+}
+
+/* The complex local energy <x|H|psi>/<x|psi> (real part: getOL); its
+ * imaginary part enters the energy gradient of a complex wave function. */
+void kvo_getol_c(int ns, int nb, const int* bonds, const int* kup, const int* kdn, const cplx* Wup,
+                 const cplx* Wdn, double* re, double* im, int* err) {
+    const cplx v = getol_sum(ns, nb, bonds, kup, kdn, Wup, Wdn, err);
+    *re = creal(v);
+    *im = cimag(v);
 }
 
 /* One chain (one walker). */

@@ -91,6 +91,8 @@ def lib():
         L.kvo_getol.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_void_p, P(C.c_int)]
         L.kvo_getol.restype = C.c_double
+        L.kvo_getol_c.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int)]
         L.kvo_getxprime.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                     P(C.c_double), C.c_void_p, C.c_void_p]
         L.kvo_philox4x32_10.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -161,10 +163,10 @@ class OracleLattice:
         ki = list(link_in.items())
         in_l1 = np.array([k[0] for k, _ in ki], dtype=np.int32)
         in_l2 = np.array([k[1] for k, _ in ki], dtype=np.int32)
-        in_v = np.array([v for _, v in ki], dtype=np.float64)
+        in_v = np.array([v for _, v in ki], dtype=np.complex128)
         kt = list(link_inter.items())
         it = [np.array([k[q] for k, _ in kt], dtype=np.int32) for q in range(4)]
-        it_v = np.array([v for _, v in kt], dtype=np.float64)
+        it_v = np.array([v for _, v in kt], dtype=np.complex128)
         H = np.zeros((self.ns, self.ns), dtype=np.complex128, order="F")
         rc = lib().kvo_hmat(C.byref(self._s), len(ki), _ptr(in_l1), _ptr(in_l2), _ptr(in_v),
                             len(kt), _ptr(it[0]), _ptr(it[1]), _ptr(it[2]), _ptr(it[3]),
@@ -254,6 +256,21 @@ def getol(bonds, kup, kdn, Wup, Wdn):
     if err.value:
         raise ValueError("site empty or doubly occupied")
     return e
+
+
+def getol_complex(bonds, kup, kdn, Wup, Wdn):
+    """The complex local energy (real part: getol)."""
+    b = bonds_array(bonds)
+    Wu = np.asfortranarray(Wup, dtype=np.complex128)
+    Wd = np.asfortranarray(Wdn, dtype=np.complex128)
+    ku = np.ascontiguousarray(kup, dtype=np.int32)
+    kd = np.ascontiguousarray(kdn, dtype=np.int32)
+    re, im, err = C.c_double(0.0), C.c_double(0.0), C.c_int(0)
+    lib().kvo_getol_c(Wu.shape[0], len(b), _ptr(b), _ptr(ku), _ptr(kd), _ptr(Wu), _ptr(Wd),
+                      C.byref(re), C.byref(im), C.byref(err))
+    if err.value:
+        raise ValueError("site empty or doubly occupied")
+    return complex(re.value, im.value)
 
 
 def getxprime(bonds, kup, kdn):
@@ -423,7 +440,7 @@ def exhaustive(bonds, Uup, Udn):
     ndn = Udn.shape[1]
     assert nup + ndn == ns
     b = bonds_array(bonds)
-    E, w, Zmu, logdet = [], [], [], []
+    E, Ec, w, Zmu, logdet = [], [], [], [], []
     confs = []
     for kup, kdn in all_configs(ns, nup):
         confs.append(tuple(int(x != 0) for x in kup))
@@ -431,9 +448,11 @@ def exhaustive(bonds, Uup, Udn):
             Wu, lu = refresh_w(Uup, kup, with_logdet=True)
             Wd, ld = refresh_w(Udn, kdn, with_logdet=True)
         except np.linalg.LinAlgError:
-            E.append(0.0); w.append(0.0); Zmu.append(z_count(b, kup, kdn)); logdet.append(-np.inf)
+            E.append(0.0); Ec.append(0.0); w.append(0.0); Zmu.append(z_count(b, kup, kdn))
+            logdet.append(-np.inf)
             continue
         E.append(getol(b, kup, kdn, Wu, Wd))
+        Ec.append(getol_complex(b, kup, kdn, Wu, Wd))
         w.append(np.exp(2.0 * (lu + ld)))
         Zmu.append(z_count(b, kup, kdn))
         logdet.append(lu + ld)
@@ -447,7 +466,8 @@ def exhaustive(bonds, Uup, Udn):
         "E": float(np.sum(w * E) / np.sum(w)),
         "E_biased": float(np.sum(wb * E) / np.sum(wb)),
         "E_quadratic_form": quad,
-        "eloc": E, "weight": w, "zmu": Zmu, "logdet": np.array(logdet), "confs": confs,
+        "eloc": E, "eloc_complex": np.array(Ec), "weight": w, "zmu": Zmu, "logdet": np.array(logdet),
+        "confs": confs,
         "psi_spin": psi,
     }
 
@@ -461,27 +481,37 @@ def log_derivs(U, dU_list, kappa):
     dense solve per parameter.  Pinned against finite differences of
     log|det tildeU(theta)|."""
     Ut = tilde_u(U, kappa)
-    out = np.zeros(len(dU_list))
+    out = np.zeros(len(dU_list), dtype=np.complex128)
     for k, dU in enumerate(dU_list):
         dUt = tilde_u(np.asarray(dU, dtype=np.complex128), kappa)
-        out[k] = float(np.real(np.trace(np.linalg.solve(Ut, dUt))))
+        out[k] = np.trace(np.linalg.solve(Ut, dUt))
+    if not (np.iscomplexobj(U) or any(np.iscomplexobj(d) for d in dU_list)):
+        return out.real
     return out
 
 
 def exhaustive_grad(bonds, Uup, Udn, dUup, dUdn):
-    """|psi|^2-weighted <E>, <O_k>, <E_L O_k> and S_kl = <O_k O_l> - <O_k><O_l>
-    over every configuration, plus grad = 2(<E_L O_k> - <E><O_k>)."""
+    """|psi|^2-weighted averages over every configuration of the local energy
+    E_L (complex for a complex psi) and the log-derivatives O_k: <E>,
+    <O_k>, <conj(O_k) E_L>, S_kl = Re(<conj(O_k) O_l> - <conj(O_k)><O_l>) and
+    grad_k = dE/dtheta_k = 2 Re(<conj(O_k) E_L> - E <conj(O_k)>), from
+    dE = 2 Re <d psi|(H - E)|psi> / <psi|psi>.  For real psi these are the
+    real formulas 2(<E_L O_k> - <E><O_k>)."""
     r = exhaustive(bonds, Uup, Udn)
     ns, nup = Uup.shape
     P = len(dUup)
-    O = np.zeros((len(r["eloc"]), P))
+    O = np.zeros((len(r["eloc"]), P), dtype=np.complex128)
     for q, (kup, kdn) in enumerate(all_configs(ns, nup)):
         if r["weight"][q] > 0:
             O[q] = log_derivs(Uup, dUup, kup) + log_derivs(Udn, dUdn, kdn)
     w = r["weight"] / r["weight"].sum()
-    E = float(w @ r["eloc"])
+    El = r["eloc_complex"]
+    E = float(np.real(w @ El))
     Om = w @ O
-    EO = (w * r["eloc"]) @ O
-    S = (O * w[:, None]).T @ O - np.outer(Om, Om)
-    r.update({"O_per_config": O, "O": Om, "EO": EO, "S": S, "grad": 2.0 * (EO - E * Om)})
+    EO = (w * El) @ O.conj()
+    S = np.real((O.conj() * w[:, None]).T @ O - np.outer(Om.conj(), Om))
+    grad = 2.0 * np.real(EO - E * Om.conj())
+    if np.all(np.imag(O) == 0) and np.all(np.imag(El) == 0):
+        O, Om, EO = O.real, Om.real, EO.real
+    r.update({"O_per_config": O, "O": Om, "EO": EO, "S": S, "grad": grad})
     return r

@@ -4,7 +4,9 @@ differences in test_gradient_host.py).
 
 Bars: 1e-9 relative for FP64 O_k and exhaustive averages; 2e-4 relative for
 O_k from FP32 W; sampled 12-site energy and gradient within 5 jackknife
-standard errors of the exact values.
+standard errors of the exact values.  theta1 carries a flux phase phi = 0.4,
+so psi, W, E_L and O_k are complex there (complex handles); theta0 is the
+real pi-flux state, whose flux-phase derivative is still complex.
 """
 import numpy as np
 import pytest
@@ -18,12 +20,17 @@ from kagome_systems import BC
 
 pytestmark = pytest.mark.gpu
 
-THETA1 = np.array([0.93, 0.07, -0.05, 0.11])
+THETA1 = np.array([0.4, 0.93, 0.07, -0.05])
 
 
 def family(L):
     pbc, apbc = BC[L]
     return ansatz_family(DoubleKagome.build(1.0, L, L, pbc, apbc))
+
+
+def orb(U, dU):
+    cplx = any(np.iscomplexobj(a) and np.any(np.imag(a) != 0) for a in [U, *dU])
+    return nat.ORB_COMPLEX if cplx else nat.ORB_REAL
 
 
 def conditioned_configs(oracle_mod, U, nw, max_cond, seed=5):
@@ -34,8 +41,8 @@ def conditioned_configs(oracle_mod, U, nw, max_cond, seed=5):
     while w < nw:
         a, b = oracle_mod.random_config(seed, 20_000 + attempt, 0, ns, n, ns - n)
         attempt += 1
-        if (np.linalg.cond(oracle_mod.tilde_u(U, a).real) < max_cond
-                and np.linalg.cond(oracle_mod.tilde_u(U, b).real) < max_cond):
+        if (np.linalg.cond(oracle_mod.tilde_u(U, a)) < max_cond
+                and np.linalg.cond(oracle_mod.tilde_u(U, b)) < max_cond):
             ku[w], kd[w] = a, b
             w += 1
     return ku, kd
@@ -46,10 +53,10 @@ def conditioned_configs(oracle_mod, U, nw, max_cond, seed=5):
 def test_measure_grad_matches_oracle(oracle_mod, L, prec):
     f = family(L)
     n = 3 * L * L // 2
-    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.G, n)
+    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.generators(THETA1), n)
     nw = 8
     ku, kd = conditioned_configs(oracle_mod, U, nw, 1e4 if prec else 1e6)
-    with nat.DeviceSampler(np.array(f.nn, dtype=np.int32), U, U, nw, precision=prec) as ds:
+    with nat.DeviceSampler(np.array(f.nn, dtype=np.int32), U, U, nw, precision=prec, orbitals=orb(U, dU)) as ds:
         ds.set_configs(ku, kd)
         ds.set_orbital_derivs(dU, dU)
         eloc, O = ds.measure_grad()
@@ -60,8 +67,9 @@ def test_measure_grad_matches_oracle(oracle_mod, L, prec):
         assert np.abs(O[w] - ref).max() < tol * max(1.0, np.abs(ref).max()), (w, O[w], ref)
         Wu = oracle_mod.refresh_w(U, ku[w])
         Wd = oracle_mod.refresh_w(U, kd[w])
-        e_ref = oracle_mod.getol(f.nn, ku[w], kd[w], Wu, Wd)
+        e_ref = oracle_mod.getol_complex(f.nn, ku[w], kd[w], Wu, Wd)
         assert abs(eloc[w] - e_ref) < tol * max(1.0, abs(e_ref))
+        assert abs(np.imag(e_ref)) > 1e-6      # complex psi: the imaginary part is live
 
 
 @pytest.mark.parametrize("prec,delay", [(0, 1), (1, 4)])
@@ -70,11 +78,11 @@ def test_measure_grad_after_sweeps(oracle_mod, prec, delay):
     measure_grad) equals the oracle's value on the walkers' new configurations."""
     L = 4
     f = family(L)
-    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.G, 24)
+    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.generators(THETA1), 24)
     nw = 16
     ku, kd = conditioned_configs(oracle_mod, U, nw, 1e4)
     with nat.DeviceSampler(np.array(f.nn, dtype=np.int32), U, U, nw, precision=prec,
-                           kernel=nat.KERNEL_UNBIASED, delay=delay, seed=3) as ds:
+                           kernel=nat.KERNEL_UNBIASED, delay=delay, seed=3, orbitals=orb(U, dU)) as ds:
         ds.set_configs(ku, kd)
         ds.set_orbital_derivs(dU, dU)
         cnt, _ = ds.sweep(37)
@@ -90,7 +98,7 @@ def test_measure_grad_after_sweeps(oracle_mod, prec, delay):
 
 def test_measure_grad_requires_derivatives():
     f = family(2)
-    U, _, _ = orbital_derivatives(f.hamiltonian(THETA0), f.G, 6)
+    U, _, _ = orbital_derivatives(f.hamiltonian(THETA0), f.generators(THETA0), 6)
     with nat.DeviceSampler(np.array(f.nn, dtype=np.int32), U, U, 2) as ds:
         ds.init_random(1)
         with pytest.raises(nat.KvmcError, match="orbital derivatives"):
@@ -100,7 +108,7 @@ def test_measure_grad_requires_derivatives():
 @pytest.mark.parametrize("theta", [THETA0, THETA1])
 def test_exhaustive_average_matches_oracle(oracle_mod, theta):
     f = family(2)
-    U, dU, _ = orbital_derivatives(f.hamiltonian(theta), f.G, 6)
+    U, dU, _ = orbital_derivatives(f.hamiltonian(theta), f.generators(theta), 6)
     ref = oracle_mod.exhaustive_grad(f.nn, U, U, dU, dU)
     got = exhaustive_average(f, theta, 6, 6)
     assert abs(got["E"] - ref["E"]) < 1e-9 * abs(ref["E"])
@@ -129,10 +137,11 @@ def _jackknife(E, O, nblocks=32):
     blocks = np.array_split(np.arange(nw), nblocks)
 
     def stats(mask):
+        # E and dE/dtheta = 2 Re(<E_L conj(O)> - E <conj(O)>) (complex psi)
         e = E[:, mask].ravel()
         o = O[:, mask].reshape(-1, O.shape[-1])
-        Em = e.mean()
-        return np.concatenate([[Em], 2.0 * ((e[:, None] * o).mean(0) - Em * o.mean(0))])
+        Em = float(np.real(e.mean()))
+        return np.concatenate([[Em], 2.0 * np.real((e[:, None] * o.conj()).mean(0) - Em * o.conj().mean(0))])
 
     full = stats(np.ones(nw, dtype=bool))
     jk = []
@@ -150,7 +159,7 @@ def test_sampled_gradient_matches_exact(oracle_mod, prec):
     """Unbiased-kernel sampling at 12 sites reproduces the exact energy and
     gradient (stationary law |psi|^2) within 5 standard errors."""
     f = family(2)
-    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.G, 6)
+    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.generators(THETA1), 6)
     ref = oracle_mod.exhaustive_grad(f.nn, U, U, dU, dU)
     gs = GradientSampler(f, THETA1, 6, 6, 2048, seed=17, precision=prec)
     try:
@@ -169,7 +178,7 @@ def test_reference_kernel_samples_biased_law(oracle_mod):
     """The reference kernel's stationary law is |psi|^2 / Z_mu: its sampled
     energy matches E_biased, not E (at 12 sites the two differ by > 10 sigma)."""
     f = family(2)
-    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA0), f.G, 6)
+    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA0), f.generators(THETA0), 6)
     ref = oracle_mod.exhaustive(f.nn, U, U)
     nw = 2048
     with nat.DeviceSampler(np.array(f.nn, dtype=np.int32), U, U, nw,

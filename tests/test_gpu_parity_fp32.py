@@ -190,9 +190,12 @@ def test_fp64_streamed_w_tape_parity_L12(oracle_mod):
     refresh cadence: bit-exact decisions and configurations, W and local
     energies to 1e-9."""
     ham = system(12)
-    nw, n = 2, 432
+    nw, n = 2, 433
     ku, kd = well_conditioned_configs(oracle_mod, 81, nw, ham)
     tu, tb = tape(nw, n, len(ham.nn), 37)
+    # u = 0 never pre-rejects: the cadence sweeps (counter % 216 == 0) reach
+    # the ratio whenever their bond is antiparallel, so refreshes happen
+    tu[:, ::ham.N_up] = 0.0
     with sampler(ham, nw) as ds:
         ds.set_configs(ku, kd)
         cnt, out = ds.sweep(n, tu, tb, record=True)

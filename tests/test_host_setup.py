@@ -106,3 +106,96 @@ def test_hamiltonian_bundle():
     assert np.allclose(ham.U_up.T @ ham.U_up, np.eye(18), atol=1e-12)
     assert np.allclose(ham.H_mat @ ham.U_up, ham.U_up * ham.spectrum[:18], atol=1e-12)
     assert ham.bonds_array().shape == (len(ham.nn), 2)
+
+
+def _loops(H):
+    """Triangles (3-cycles) and hexagons (chordless 6-cycles) of the bond
+    graph of H."""
+    n = H.shape[0]
+    nbrs = [set(np.nonzero(H[i])[0]) - {i} for i in range(n)]
+    tri = set()
+    for i in range(n):
+        for j in nbrs[i]:
+            for k in nbrs[j] & nbrs[i]:
+                tri.add(tuple(sorted((i, j, k))))
+    hexes = {}
+    for a in range(n):
+        for b in nbrs[a]:
+            for c in nbrs[b] - {a}:
+                if c in nbrs[a]:
+                    continue
+                for d in nbrs[c] - {a, b}:
+                    if d in nbrs[b]:
+                        continue
+                    for e in nbrs[d] - {a, b, c}:
+                        if e in nbrs[c]:
+                            continue
+                        for f in (nbrs[e] & nbrs[a]) - {b, c, d}:
+                            if f in nbrs[d] or f in nbrs[b]:
+                                continue
+                            hexes.setdefault(frozenset((a, b, c, d, e, f)), (a, b, c, d, e, f))
+    return [list(t) for t in tri], list(hexes.values())
+
+
+def _ccw(lat, pos, cyc):
+    """The cycle in counterclockwise order (minimal-image steps on the torus)."""
+    T1, T2 = (lat.n1 // 2) * lat.a1, lat.n2 * lat.a2
+    p = np.zeros(2)
+    area = 0.0
+    for q in range(len(cyc)):
+        d = pos[cyc[(q + 1) % len(cyc)]] - pos[cyc[q]]
+        best = min((d + i * T1 + j * T2 for i in (-1, 0, 1) for j in (-1, 0, 1)), key=np.linalg.norm)
+        area += p[0] * best[1] - p[1] * best[0]
+        p = p + best
+    return cyc if area > 0 else cyc[::-1]
+
+
+@pytest.mark.parametrize("phi", [np.pi / 3, 0.9, 0.0])
+def test_flux_family_pattern(phi):
+    """flux_link_tables(phi): the phase of the product of t_xy = -H[x, y]
+    along every counterclockwise triangle is phi and along every
+    counterclockwise hexagon pi - 2 phi (mod 2 pi); |t| = 1 on every bond;
+    phi = 0 gives the reference's pi-flux H exactly; the derivative tables
+    equal a central difference of H in phi."""
+    from kagome_vmc.hamiltonian import flux_link_tables
+    from kagome_vmc.lattice import site_coords
+    lat = DoubleKagome.build(1.0, 8, 8, (True, True), (False, False))
+    H = Hmat(lat, *flux_link_tables(lat, phi))
+    assert np.abs(H - H.conj().T).max() < 1e-15
+    nz = H[H != 0]
+    assert np.allclose(np.abs(nz), 1.0)
+    if phi == 0.0:
+        assert np.array_equal(H, Hmat(lat))
+    pos = site_coords(lat)
+    tri, hexes = _loops(H)
+    assert len(tri) == 2 * H.shape[0] // 3 and len(hexes) == H.shape[0] // 3
+
+    def phase(cyc):
+        c = _ccw(lat, pos, cyc)
+        prod = 1.0 + 0j
+        for q in range(len(c)):
+            prod *= -H[c[q], c[(q + 1) % len(c)]]
+        return np.angle(prod)
+
+    wrap = lambda x: (x + np.pi) % (2 * np.pi) - np.pi
+    for t in tri:
+        assert abs(wrap(phase(t) - phi)) < 1e-12
+    for h in hexes:
+        assert abs(wrap(phase(h) - (np.pi - 2 * phi))) < 1e-12
+    eps = 1e-6
+    dH = Hmat(lat, *flux_link_tables(lat, phi, dphi=True))
+    fd = (Hmat(lat, *flux_link_tables(lat, phi + eps)) - Hmat(lat, *flux_link_tables(lat, phi - eps))) / (2 * eps)
+    assert np.abs(dH - fd).max() < 1e-9
+
+
+def test_flux_family_matches_oracle(oracle_mod):
+    """The complex flux tables through the oracle's Hmat (complex link
+    amplitudes, plus a field B) give the product's H."""
+    from kagome_vmc.hamiltonian import flux_link_tables
+    for (pbc, apbc, B) in (((True, True), (False, True), 0.0), ((True, True), (False, True), 0.1),
+                           ((True, False), (False, False), 0.05)):
+        lat = DoubleKagome.build(1.0, 4, 4, pbc, apbc)
+        tin, tint = flux_link_tables(lat, np.pi / 3)
+        Hp = Hmat(lat, tin, tint, B=B)
+        Ho = oracle_mod.OracleLattice(1.0, 4, 4, pbc, apbc).hmat(tin, tint, B=B)
+        assert np.abs(Hp - Ho).max() < 1e-13

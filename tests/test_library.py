@@ -48,8 +48,9 @@ def test_version_and_error_string(lib):
 
 def test_struct_layout_matches_header(lib):
     from kagome_vmc import _native
-    # kvmc_desc: 4 int32, 3 pointers, int64, 6 int32, uint64 (x86-64 layout)
-    assert ctypes.sizeof(_native.KvmcDesc) == 4 * 4 + 3 * 8 + 8 + 6 * 4 + 8
+    # kvmc_desc: 4 int32, 3 pointers, int64, 6 int32, uint64, int32 + 4 bytes
+    # of tail padding (x86-64 layout)
+    assert ctypes.sizeof(_native.KvmcDesc) == 4 * 4 + 3 * 8 + 8 + 6 * 4 + 8 + 4 + 4
     assert ctypes.sizeof(_native.KvmcCounters) == 6 * 8
 
 
