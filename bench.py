@@ -372,39 +372,62 @@ MIXED_GROUPS = ((4, 4096), (6, 2048), (8, 2048), (10, 1024), (12, 1024))
 def run_mixed(args):
     """All groups of MIXED_GROUPS (L, walkers) at once, one handle and host
     thread per group, FP32 W with delay and FP64 refresh every `k` accepts.
-    One step = n_occupied sweeps + one local-energy pass in every group.
-    Also times each group alone, for the overlap gain."""
+    Each lattice takes a flux phase phi drawn uniformly from [0, pi] with the
+    seed (complex orbitals, complex W).  One step = n_occupied sweeps + one
+    local-energy pass in every group.  Also times each group alone, for the
+    overlap gain and for each group's kernel times and HBM roofline."""
     from kagome_vmc import _native as nat
-    from kagome_vmc.batch import MixedBatch, pi_flux_cluster
+    from kagome_vmc.batch import MixedBatch, flux_cluster
     nat.load()
-    groups = [(pi_flux_cluster(L), nw) for L, nw in MIXED_GROUPS]
+    rng = np.random.default_rng(args.seed)
+    phis = [float(rng.uniform(0.0, np.pi)) for _ in MIXED_GROUPS]
+    groups = [(flux_cluster(L, phi), nw) for (L, nw), phi in zip(MIXED_GROUPS, phis)]
     kw = dict(precision=nat.PREC_FP32, refresh_policy=nat.REFRESH_EVERY_K, refresh_k=args.k,
               delay=args.delay, seed=args.seed)
 
-    def timed(batch):
+    def timed(batch, profile=False):
         batch.init_random(args.seed)
+        for g in batch.groups:
+            g.sampler.stagger_refresh(args.seed)
         batch.step(args.warmup)
         sync()
+        if profile:
+            for g in batch.groups:
+                g.sampler.profile(True)
         t0 = time.perf_counter()
         res = batch.step(args.steps)
         sync()
-        return time.perf_counter() - t0, res
+        wall = time.perf_counter() - t0
+        prof = [g.sampler.profile() for g in batch.groups] if profile else None
+        return wall, res, prof
 
+    # all groups in one graph launch per step (kvmc_group_sweep); with
+    # --threads-mixed one host thread per group instead
+    with MixedBatch(groups, graph=not args.threads_mixed, **kw) as mb:
+        wall, res, _ = timed(mb)
     with MixedBatch(groups, **kw) as mb:
-        if args.graph:
-            for g in mb.groups:
-                g.sampler.enable_graph(True)
-        wall, res = timed(mb)
+        wall_threads, res_threads, _ = timed(mb)
     per = []
     alone_total = 0.0
-    for (ham, nw), (cnt, _) in zip(groups, res):
+    for (ham, nw), (cnt, _), phi in zip(groups, res, phis):
         with MixedBatch([(ham, nw)], **kw) as one:
-            if args.graph:
-                one.groups[0].sampler.enable_graph(True)
-            t1, _ = timed(one)
+            t1, r1, prof = timed(one, profile=True)
         alone_total += t1
-        per.append({"L": int(round((ham.ns / 3) ** 0.5)), "walkers": nw, "walker_sweeps": cnt["sweeps"],
-                    "alone_walker_sweeps_per_s": cnt["sweeps"] / t1, "refreshes": cnt["refreshes"]})
+        c1 = r1[0][0]
+        s = 8   # complex float W elements
+        b = sweep_bytes(c1, s, ham.ns, ham.N_up, ham.N_down, nw)
+        sweep_gbs = b / (prof[0]["sweep_ms"] * 1e-3) / 1e9 if prof[0]["sweep_ms"] > 0 else None
+        per.append({"L": int(round((ham.ns / 3) ** 0.5)), "walkers": nw, "phi": phi,
+                    "walker_sweeps": cnt["sweeps"],
+                    "alone_walker_sweeps_per_s": c1["sweeps"] / t1, "refreshes": cnt["refreshes"],
+                    # both W of a walker within 64 KiB stay in LDS (empty-site rows only)
+                    "resident_w_in_lds": ((ham.ns - ham.N_up + 3) // 4 * 4) * (ham.N_up + ham.N_down) * s
+                                         <= 64 * 1024,
+                    "alone_kernel_ms": {k2: prof[0][k2] for k2 in ("sweep_ms", "refresh_ms", "measure_ms")},
+                    "alone_sweep_gbs": sweep_gbs,
+                    "alone_sweep_hbm_frac": (sweep_gbs / HBM_PEAK_GBS) if sweep_gbs else None})
+        log(f"[mixed] L={per[-1]['L']} phi={phi:.3f}: alone {per[-1]['alone_walker_sweeps_per_s']:.3e} "
+            f"walker-sweeps/s, sweep kernel {sweep_gbs or 0:.0f} GB/s")
     total = sum(c["sweeps"] for c, _ in res)
     line = {
         "metric": "walker-sweeps/s, mixed cluster sizes L in {4,6,8,10,12} on one GPU",
@@ -412,9 +435,12 @@ def run_mixed(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "mixed: " + ", ".join(f"L={L} x {nw}" for L, nw in MIXED_GROUPS)
-                               + f"; FP32 W delay {args.delay}, FP64 refresh every {args.k} accepts",
+                               + f"; seeded flux phase phi in [0, pi] per lattice (complex W); FP32 W delay "
+                               f"{args.delay}, FP64 refresh every {args.k} accepts",
                    "groups": per},
         "sequential_walker_sweeps_per_s": total / alone_total,
+        "launch": "one graph launch per step for all groups" if not args.threads_mixed else "one host thread per group",
+        "threads_walker_sweeps_per_s": sum(c["sweeps"] for c, _ in res_threads) / wall_threads,
     }
     print(json.dumps(line), flush=True)
 
@@ -503,6 +529,8 @@ def main():
                     help="only run the gradient workload (unbiased kernel, FP64, log-derivatives)")
     ap.add_argument("--mixed", action="store_true",
                     help="only run the mixed-cluster-size batch (L = 4..12 groups at once)")
+    ap.add_argument("--threads-mixed", action="store_true",
+                    help="mixed batch: one host thread per group instead of one graph launch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()

@@ -77,6 +77,7 @@ extern "C" {
 #define KVMC_OUT_ACCEPT 3     /* accepted move (:581-591) */
 
 typedef struct kvmc_handle kvmc_handle;
+typedef struct kvmc_group kvmc_group;
 
 typedef struct kvmc_desc {
     int32_t ns;             /* sites */
@@ -145,6 +146,22 @@ int kvmc_refresh(kvmc_handle* h);
  * per sweep.  counters (optional) receives this call's totals. */
 int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
                int8_t* outcomes, kvmc_counters* counters);
+
+/* No counterpart in the reference, whose runs are one lattice each (Carlo
+ * farms separate runs).  A group of handles on one GPU -- e.g. lattices of
+ * different sizes, one handle each -- whose sweeps run as one graph launch:
+ * kvmc_group_sweep captures every handle's sweep sequence (as replayed by
+ * kvmc_enable_graph) as a parallel branch of one hipGraph, on the handles'
+ * own streams, so each size keeps its own kernel shapes and LDS footprint,
+ * then replays it with one launch per call.  Walkers still paused after the
+ * captured rounds are finished by each handle's ordinary launch loop.
+ * Results equal kvmc_sweep on each handle.  A handle that is profiling or
+ * tracking drift makes the call sweep the handles one after another.
+ * The group borrows the handles: destroy it before them. */
+int kvmc_group_create(kvmc_handle* const* handles, int32_t n, kvmc_group** out);
+/* nsweeps: one count per handle (0 skips it); counters (optional): one per handle. */
+int kvmc_group_sweep(kvmc_group* g, const int64_t* nsweeps, kvmc_counters* counters);
+int kvmc_group_destroy(kvmc_group* g);
 
 /* getOL for every walker (src/Hamiltonian.jl:762-778): Heisenberg J1 local
  * energy real(<x|H|psi>/<x|psi>), accumulated in FP64.  eloc: nwalkers

@@ -17,6 +17,7 @@
 
 using namespace kvmc;
 
+
 namespace {
 
 thread_local std::string g_err;
@@ -118,6 +119,7 @@ __global__ void k_fill_ull(unsigned long long* p, unsigned long long v, int n) {
     if (i < n) p[i] = v;
 }
 
+
 // Relative drift of each listed walker (max |W - W_fresh| / max |W_fresh|
 // over both spins, from the per-walker maxima of a drift pass) folded into
 // stats[0] (maximum, ordered bits), stats[1] (sum) and stats[2] (count).
@@ -209,6 +211,16 @@ struct kvmc_handle {
     hipGraphExec_t gexec = nullptr;
     int64_t g_nsweeps = -1;
     kvmc_profile prof{};
+};
+
+// Several handles on one GPU swept by one graph launch (kvmc_group_*).
+struct kvmc_group {
+    std::vector<kvmc_handle*> hs;
+    hipStream_t s0 = nullptr;          // the stream the graph is captured on and launched on
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int64_t> g_nsweeps;    // the sweep counts the graph was captured for
 };
 
 namespace {
@@ -646,145 +658,109 @@ int kvmc_refresh(kvmc_handle* h) {
     return refresh_all(h, 1);
 }
 
-int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
-               int8_t* outcomes, kvmc_counters* counters) {
-    g_err.clear();
+}  // extern "C"
+
+namespace {
+
+// Argument checks shared by the sweep entry points.
+int sweep_check(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond) {
     if (!h) return fail(KVMC_ERR_INVALID, "null handle");
     if (!h->ready) return fail(KVMC_ERR_NOT_READY, "configurations not set");
     if (nsweeps < 0) return fail(KVMC_ERR_INVALID, "nsweeps < 0");
     if ((tape_u == nullptr) != (tape_bond == nullptr))
         return fail(KVMC_ERR_INVALID, "tape_u and tape_bond go together");
-    if (nsweeps == 0) { if (counters) *counters = kvmc_counters{}; return KVMC_OK; }
-    const size_t tape_n = (size_t)h->nw * (size_t)nsweeps;
+    return KVMC_OK;
+}
+
+int tape_check(const kvmc_handle* h, size_t tape_n, const double* tape_u, const int32_t* tape_bond) {
     if (tape_u)
         for (size_t q = 0; q < tape_n; ++q)
             if (tape_bond[q] < 0 || tape_bond[q] >= h->nb || !(tape_u[q] >= 0.0 && tape_u[q] < 1.0))
                 return fail(KVMC_ERR_INVALID, "tape entry " + std::to_string(q) + " out of range");
-    double* d_tu = nullptr;
-    int* d_tb = nullptr;
-    signed char* d_out = nullptr;
-    int rc = KVMC_OK;
-    auto cleanup = [&]() {
-        if (d_tu) (void)hipFree(d_tu);
-        if (d_tb) (void)hipFree(d_tb);
-        if (d_out) (void)hipFree(d_out);
-    };
-    if (tape_u) {
-        if ((rc = dalloc(&d_tu, tape_n)) || (rc = dalloc(&d_tb, tape_n))) { cleanup(); return rc; }
-        if (hipMemcpyAsync(d_tu, tape_u, tape_n * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-            hipMemcpyAsync(d_tb, tape_bond, tape_n * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
-            cleanup();
-            return fail(KVMC_ERR_HIP, "tape upload failed");
+    return KVMC_OK;
+}
+
+// Enqueues one captured sweep sequence of handle h on stream s: stop counts,
+// counter and flag resets (fill kernels, no memset nodes), then
+// kGraphRounds rounds of the sweep kernel, each but the last followed by
+// the refresh of the walkers it paused, their count read on the device.
+hipError_t enqueue_rounds(kvmc_handle* h, const SweepArgs& a, int threads, size_t lds, int64_t nsweeps,
+                          hipStream_t s) {
+    const int fb = (h->nw + 255) / 256;
+    hipLaunchKernelGGL(k_set_stop, dim3(fb), dim3(256), 0, s, h->sweeps, h->stop, h->tbase, (long long)nsweeps,
+                       h->nw);
+    hipLaunchKernelGGL(k_fill_ull, dim3(1), dim3(64), 0, s, h->counters, 0ull, (int)CNT_SLOTS);
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->failed, INT_MAX, 1);
+    hipError_t ce = hipGetLastError();
+    for (int r = 0; r < kGraphRounds && ce == hipSuccess; ++r) {
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->rcount, 0, 1);
+        ce = do_sweep(h, a, h->nw, threads, lds, s);
+        if (r + 1 == kGraphRounds || ce != hipSuccess) break;
+        RefreshArgs ra = refresh_args(h, 1, 0.0);
+        ce = prepare_refresh(h, s);
+        const int batch = std::max(1, h->ws_jobs / 2);
+        for (int b0 = 0; b0 < h->nw && ce == hipSuccess; b0 += batch) {
+            ra.jobs = h->rlist + b0;
+            ra.njobs = 2 * std::min(batch, h->nw - b0);
+            ra.count_dev = h->rcount;
+            ra.walker_off = b0;
+            ce = launch_refresh(h->prec, ra, s);
         }
     }
-    if (outcomes && (rc = dalloc(&d_out, tape_n))) { cleanup(); return rc; }
-    SweepArgs a = sweep_args(h);
-    a.tape_u = d_tu; a.tape_bond = d_tb; a.tape_len = nsweeps; a.outcomes = d_out;
-    a.walker0 = 0;
-    const int threads = sweep_threads(h);
-    const size_t lds = sweep_lds_bytes(h->esize, h->ns, h->nup, h->ndn, h->delay, h->nb);
-    if (lds > 160 * 1024) { cleanup(); return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay"); }
-    int64_t launches = 0;
+    return ce;
+}
+
+// After a replay of enqueue_rounds: reports a singular refresh, refreshes
+// the walkers the last round paused; *finished when nothing is left to run.
+int after_replay(kvmc_handle* h, bool* finished) {
     hipStream_t s = h->stream;
-    const bool use_graph = h->graph && !tape_u && !outcomes && !h->profiling && !h->track_drift;
-    bool finished = false;   // the replay left no walker paused
-    if (use_graph) {
-        // One replay: stop counts, counter and flag resets (fill kernels, no
-        // memset nodes), then kGraphRounds rounds of sweep + refresh of the
-        // paused walkers with the count read on the device.
-        if (!h->gexec || h->g_nsweeps != nsweeps) {
-            if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-            hipGraph_t graph = nullptr;
-            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-            const int fb = (h->nw + 255) / 256;
-            hipLaunchKernelGGL(k_set_stop, dim3(fb), dim3(256), 0, s, h->sweeps, h->stop, h->tbase,
-                               (long long)nsweeps, h->nw);
-            hipLaunchKernelGGL(k_fill_ull, dim3(1), dim3(64), 0, s, h->counters, 0ull, (int)CNT_SLOTS);
-            hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->failed, INT_MAX, 1);
-            hipError_t ce = hipSuccess;
-            for (int r = 0; r < kGraphRounds && ce == hipSuccess; ++r) {
-                hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, h->rcount, 0, 1);
-                ce = do_sweep(h, a, h->nw, threads, lds, s);
-                if (r + 1 == kGraphRounds || ce != hipSuccess) break;
-                RefreshArgs ra = refresh_args(h, 1, 0.0);
-                if (ce == hipSuccess) ce = prepare_refresh(h, s);
-                const int batch = std::max(1, h->ws_jobs / 2);
-                for (int b0 = 0; b0 < h->nw && ce == hipSuccess; b0 += batch) {
-                    ra.jobs = h->rlist + b0;
-                    ra.njobs = 2 * std::min(batch, h->nw - b0);
-                    ra.count_dev = h->rcount;
-                    ra.walker_off = b0;
-                    ce = launch_refresh(h->prec, ra, s);
-                }
-            }
-            hipError_t ee = hipStreamEndCapture(s, &graph);
-            if (ce != hipSuccess || ee != hipSuccess) {
-                if (graph) (void)hipGraphDestroy(graph);
-                cleanup();
-                return fail(KVMC_ERR_HIP, std::string("sweep graph capture: ") +
-                                              hipGetErrorString(ce != hipSuccess ? ce : ee));
-            }
-            hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) { h->gexec = nullptr; cleanup(); return fail(KVMC_ERR_HIP, "sweep graph instantiate failed"); }
-            h->g_nsweeps = nsweeps;
-        }
-        HIP_TRY(hipGraphLaunch(h->gexec, s));
-        launches += kGraphRounds;
-        int count = 0, failed = INT_MAX;
-        HIP_TRY(hipMemcpyAsync(&count, h->rcount, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&failed, h->failed, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (failed != INT_MAX) {
-            h->failed_walker = failed;
-            cleanup();
-            return fail(KVMC_ERR_SINGULAR, "tildeU is singular for walker " + std::to_string(failed) +
-                                               " (lu factorization failed)");
-        }
-        if (count == 0) {
-            finished = true;
-        } else {
-            rc = refresh_list(h, h->rlist, count, 1, 0.0, h->track_drift);
-            if (rc) { cleanup(); return rc; }
-            finished = h->policy == KVMC_REFRESH_AT_END;
-        }
-    } else {
-        hipLaunchKernelGGL(k_set_stop, dim3((h->nw + 255) / 256), dim3(256), 0, s,
-                           h->sweeps, h->stop, h->tbase, (long long)nsweeps, h->nw);
-        if (hipMemsetAsync(h->counters, 0, sizeof(unsigned long long) * CNT_SLOTS, s) != hipSuccess) {
-            cleanup(); return fail(KVMC_ERR_HIP, "counter reset failed");
-        }
+    int count = 0, failed = INT_MAX;
+    HIP_TRY(hipMemcpyAsync(&count, h->rcount, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&failed, h->failed, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (failed != INT_MAX) {
+        h->failed_walker = failed;
+        return fail(KVMC_ERR_SINGULAR, "tildeU is singular for walker " + std::to_string(failed) +
+                                           " (lu factorization failed)");
     }
+    *finished = count == 0;
+    if (count > 0) {
+        int rc = refresh_list(h, h->rlist, count, 1, 0.0, h->track_drift);
+        if (rc) return rc;
+        *finished = h->policy == KVMC_REFRESH_AT_END;
+    }
+    return KVMC_OK;
+}
+
+// The ordinary launch loop (sweep launch, refresh of the paused walkers,
+// again until no walker pauses), then the outcome copy and the counters.
+int sweep_tail(kvmc_handle* h, const SweepArgs& a, int threads, size_t lds, bool finished, int64_t launches,
+               int8_t* outcomes, const signed char* d_out, size_t tape_n, kvmc_counters* counters) {
+    hipStream_t s = h->stream;
     while (!finished) {
         int count = 0;
-        if (hipMemsetAsync(h->rcount, 0, sizeof(int), s) != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, "hipMemsetAsync failed"); }
+        if (hipMemsetAsync(h->rcount, 0, sizeof(int), s) != hipSuccess) return fail(KVMC_ERR_HIP, "hipMemsetAsync failed");
         prof_begin(h);
         hipError_t e = do_sweep(h, a, h->nw, threads, lds, s);
-        if (e != hipSuccess) { cleanup(); return fail(KVMC_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e)); }
+        if (e != hipSuccess) return fail(KVMC_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
         prof_end(h);
         ++launches;
         if (hipMemcpyAsync(&count, h->rcount, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            cleanup();
+            hipStreamSynchronize(s) != hipSuccess)
             return fail(KVMC_ERR_HIP, std::string("sweep kernel: ") + hipGetErrorString(hipGetLastError()));
-        }
         h->prof.sweep_ms += prof_elapsed(h);
         h->prof.sweep_launches += h->profiling;
         if (count == 0) break;
-        rc = refresh_list(h, h->rlist, count, 1, 0.0, h->track_drift);
-        if (rc) { cleanup(); return rc; }
+        int rc = refresh_list(h, h->rlist, count, 1, 0.0, h->track_drift);
+        if (rc) return rc;
         if (h->policy == KVMC_REFRESH_AT_END) break;   // listed walkers had finished
     }
-    if (outcomes &&
-        hipMemcpyAsync(outcomes, d_out, tape_n, hipMemcpyDeviceToHost, s) != hipSuccess) {
-        cleanup(); return fail(KVMC_ERR_HIP, "outcome copy failed");
-    }
+    if (outcomes && hipMemcpyAsync(outcomes, d_out, tape_n, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return fail(KVMC_ERR_HIP, "outcome copy failed");
     unsigned long long cnt[CNT_SLOTS] = {};
     if (hipMemcpyAsync(cnt, h->counters, sizeof(cnt), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        cleanup(); return fail(KVMC_ERR_HIP, "counter copy failed");
-    }
-    cleanup();
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(KVMC_ERR_HIP, "counter copy failed");
     if (counters) {
         counters->sweeps = (int64_t)cnt[CNT_SWEEPS];
         counters->moves = (int64_t)cnt[CNT_MOVES];
@@ -794,6 +770,189 @@ int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int3
         counters->launches = launches;
     }
     return KVMC_OK;
+}
+
+// The sweep calls with the tape and the outcome record staged in the given
+// device buffers (each used only when its host pointer is set): uploads the
+// tape, runs the launch / refresh loop (or the captured graph when
+// allow_graph), copies the outcomes back and reads the counters.  Allocates
+// no device memory.
+int sweep_staged(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
+                 int8_t* outcomes, kvmc_counters* counters, double* d_tu, int* d_tb, signed char* d_out,
+                 bool allow_graph) {
+    const size_t tape_n = (size_t)h->nw * (size_t)nsweeps;
+    if (tape_u) {
+        if (hipMemcpyAsync(d_tu, tape_u, tape_n * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(d_tb, tape_bond, tape_n * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+            return fail(KVMC_ERR_HIP, "tape upload failed");
+    }
+    SweepArgs a = sweep_args(h);
+    a.tape_u = tape_u ? d_tu : nullptr; a.tape_bond = tape_u ? d_tb : nullptr; a.tape_len = nsweeps;
+    a.outcomes = outcomes ? d_out : nullptr;
+    a.walker0 = 0;
+    const int threads = sweep_threads(h);
+    const size_t lds = sweep_lds_bytes(h->esize, h->ns, h->nup, h->ndn, h->delay, h->nb);
+    if (lds > 160 * 1024) return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay");
+    int64_t launches = 0;
+    hipStream_t s = h->stream;
+    const bool use_graph = allow_graph && h->graph && !tape_u && !outcomes && !h->profiling && !h->track_drift;
+    bool finished = false;   // the replay left no walker paused
+    if (use_graph) {
+        if (!h->gexec || h->g_nsweeps != nsweeps) {
+            if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            hipError_t ce = enqueue_rounds(h, a, threads, lds, nsweeps, s);
+            hipError_t ee = hipStreamEndCapture(s, &graph);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(KVMC_ERR_HIP, std::string("sweep graph capture: ") +
+                                              hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            hipError_t ie = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) { h->gexec = nullptr; return fail(KVMC_ERR_HIP, "sweep graph instantiate failed"); }
+            h->g_nsweeps = nsweeps;
+        }
+        HIP_TRY(hipGraphLaunch(h->gexec, s));
+        launches += kGraphRounds;
+        int rc = after_replay(h, &finished);
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL(k_set_stop, dim3((h->nw + 255) / 256), dim3(256), 0, s,
+                           h->sweeps, h->stop, h->tbase, (long long)nsweeps, h->nw);
+        if (hipMemsetAsync(h->counters, 0, sizeof(unsigned long long) * CNT_SLOTS, s) != hipSuccess)
+            return fail(KVMC_ERR_HIP, "counter reset failed");
+    }
+    return sweep_tail(h, a, threads, lds, finished, launches, outcomes, d_out, tape_n, counters);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kvmc_group_create(kvmc_handle* const* handles, int32_t n, kvmc_group** out) {
+    g_err.clear();
+    if (!handles || !out || n < 1) return fail(KVMC_ERR_INVALID, "bad group arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!handles[i]) return fail(KVMC_ERR_INVALID, "null handle in group");
+        if (handles[i]->device != handles[0]->device) return fail(KVMC_ERR_INVALID, "group handles on different devices");
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == handles[i]) return fail(KVMC_ERR_INVALID, "handle listed twice in group");
+    }
+    kvmc_group* g = new kvmc_group();
+    g->hs.assign(handles, handles + n);
+    g->join.assign(n, nullptr);
+    bool ok = hipStreamCreate(&g->s0) == hipSuccess &&
+              hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < n; ++i) ok = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        kvmc_group_destroy(g);
+        return fail(KVMC_ERR_HIP, "group stream or event creation failed");
+    }
+    *out = g;
+    return KVMC_OK;
+}
+
+int kvmc_group_destroy(kvmc_group* g) {
+    if (!g) return KVMC_OK;
+    if (g->s0) (void)hipStreamSynchronize(g->s0);
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+    for (hipEvent_t e : g->join)
+        if (e) (void)hipEventDestroy(e);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->s0) (void)hipStreamDestroy(g->s0);
+    delete g;
+    return KVMC_OK;
+}
+
+int kvmc_group_sweep(kvmc_group* g, const int64_t* nsweeps, kvmc_counters* counters) {
+    g_err.clear();
+    if (!g || !nsweeps) return fail(KVMC_ERR_INVALID, "null group or sweep counts");
+    const int n = (int)g->hs.size();
+    bool plain = false;   // profiling or drift tracking: each handle sweeps on its own
+    for (int i = 0; i < n; ++i) {
+        int rc = sweep_check(g->hs[i], nsweeps[i], nullptr, nullptr);
+        if (rc) return rc;
+        plain = plain || g->hs[i]->profiling || g->hs[i]->track_drift;
+    }
+    if (plain) {
+        for (int i = 0; i < n; ++i) {
+            int rc = nsweeps[i] ? sweep_staged(g->hs[i], nsweeps[i], nullptr, nullptr, nullptr,
+                                               counters ? counters + i : nullptr, nullptr, nullptr, nullptr, false)
+                                : KVMC_OK;
+            if (rc) return rc;
+            if (!nsweeps[i] && counters) counters[i] = kvmc_counters{};
+        }
+        return KVMC_OK;
+    }
+    std::vector<SweepArgs> args(n);
+    std::vector<int> threads(n);
+    std::vector<size_t> lds(n);
+    for (int i = 0; i < n; ++i) {
+        kvmc_handle* h = g->hs[i];
+        args[i] = sweep_args(h);
+        threads[i] = sweep_threads(h);
+        lds[i] = sweep_lds_bytes(h->esize, h->ns, h->nup, h->ndn, h->delay, h->nb);
+        if (lds[i] > 160 * 1024) return fail(KVMC_ERR_INVALID, "LDS footprint exceeds 160 KiB; lower delay");
+    }
+    if (!g->gexec || g->g_nsweeps != std::vector<int64_t>(nsweeps, nsweeps + n)) {
+        if (g->gexec) { (void)hipGraphExecDestroy(g->gexec); g->gexec = nullptr; }
+        // fork: every handle's stream joins the capture of s0, enqueues its
+        // rounds, and s0 joins back on its event
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(g->s0, hipStreamCaptureModeRelaxed));
+        hipError_t ce = hipEventRecord(g->fork, g->s0);
+        for (int i = 0; i < n && ce == hipSuccess; ++i) {
+            if (!nsweeps[i]) continue;
+            kvmc_handle* h = g->hs[i];
+            ce = hipStreamWaitEvent(h->stream, g->fork, 0);
+            if (ce == hipSuccess) ce = enqueue_rounds(h, args[i], threads[i], lds[i], nsweeps[i], h->stream);
+            if (ce == hipSuccess) ce = hipEventRecord(g->join[i], h->stream);
+            if (ce == hipSuccess) ce = hipStreamWaitEvent(g->s0, g->join[i], 0);
+        }
+        hipError_t ee = hipStreamEndCapture(g->s0, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(KVMC_ERR_HIP, std::string("group graph capture: ") +
+                                          hipGetErrorString(ce != hipSuccess ? ce : ee));
+        }
+        hipError_t ie = hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) { g->gexec = nullptr; return fail(KVMC_ERR_HIP, "group graph instantiate failed"); }
+        g->g_nsweeps.assign(nsweeps, nsweeps + n);
+    }
+    HIP_TRY(hipGraphLaunch(g->gexec, g->s0));
+    HIP_TRY(hipStreamSynchronize(g->s0));
+    // walkers the last round paused finish on their own handle's stream
+    for (int i = 0; i < n; ++i) {
+        kvmc_handle* h = g->hs[i];
+        if (!nsweeps[i]) { if (counters) counters[i] = kvmc_counters{}; continue; }
+        bool finished = false;
+        int rc = after_replay(h, &finished);
+        if (rc) return rc;
+        rc = sweep_tail(h, args[i], threads[i], lds[i], finished, kGraphRounds, nullptr, nullptr, 0,
+                        counters ? counters + i : nullptr);
+        if (rc) return rc;
+    }
+    return KVMC_OK;
+}
+
+int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
+               int8_t* outcomes, kvmc_counters* counters) {
+    g_err.clear();
+    int rc = sweep_check(h, nsweeps, tape_u, tape_bond);
+    if (rc) return rc;
+    if (nsweeps == 0) { if (counters) *counters = kvmc_counters{}; return KVMC_OK; }
+    const size_t tape_n = (size_t)h->nw * (size_t)nsweeps;
+    if ((rc = tape_check(h, tape_n, tape_u, tape_bond))) return rc;
+    DevBuf<double> d_tu;
+    DevBuf<int> d_tb;
+    DevBuf<signed char> d_out;
+    if (tape_u && ((rc = d_tu.alloc(tape_n)) || (rc = d_tb.alloc(tape_n)))) return rc;
+    if (outcomes && (rc = d_out.alloc(tape_n))) return rc;
+    return sweep_staged(h, nsweeps, tape_u, tape_bond, outcomes, counters, d_tu.p, d_tb.p, d_out.p, true);
 }
 
 static ElocArgs eloc_args(kvmc_handle* h) {

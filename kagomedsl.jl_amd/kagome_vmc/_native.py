@@ -27,6 +27,7 @@ EXPORTS = [
     "kvmc_get_sweep_count", "kvmc_w_drift", "kvmc_failed_walker", "kvmc_last_error",
     "kvmc_version", "kvmc_profile_enable", "kvmc_get_profile", "kvmc_stagger_refresh",
     "kvmc_enable_graph", "kvmc_track_drift", "kvmc_get_drift_stats",
+    "kvmc_group_create", "kvmc_group_sweep", "kvmc_group_destroy",
 ]
 
 
@@ -90,6 +91,9 @@ def load(path: str = LIB_PATH):
         "kvmc_init_random": ([vp, C.c_uint64, C.c_int32, P(C.c_int64)], C.c_int),
         "kvmc_refresh": ([vp], C.c_int),
         "kvmc_sweep": ([vp, C.c_int64, vp, vp, vp, P(KvmcCounters)], C.c_int),
+        "kvmc_group_create": ([vp, C.c_int32, P(vp)], C.c_int),
+        "kvmc_group_sweep": ([vp, vp, vp], C.c_int),
+        "kvmc_group_destroy": ([vp], C.c_int),
         "kvmc_measure": ([vp, vp, P(C.c_double), P(C.c_int64)], C.c_int),
         "kvmc_set_orbital_derivs": ([vp, C.c_int32, vp, vp], C.c_int),
         "kvmc_measure_grad": ([vp, vp, vp], C.c_int),
@@ -308,3 +312,43 @@ class DeviceSampler:
 
     def failed_walker(self):
         return int(load().kvmc_failed_walker(self._h))
+
+
+class SamplerGroup:
+    """Several DeviceSamplers on one GPU (e.g. lattices of different sizes)
+    whose sweeps run as one graph launch per call (kvmc_group_sweep): each
+    sampler's sweep sequence is a parallel branch of one captured graph.
+    Borrows the samplers; close it before them."""
+
+    def __init__(self, samplers):
+        self.samplers = list(samplers)
+        arr = (C.c_void_p * len(self.samplers))(*[s._h.value for s in self.samplers])
+        g = C.c_void_p()
+        _check(load().kvmc_group_create(arr, len(self.samplers), C.byref(g)))
+        self._g = g
+
+    def sweep(self, nsweeps):
+        """nsweeps: one count per sampler.  Returns one counter dict each."""
+        n = np.ascontiguousarray(nsweeps, dtype=np.int64)
+        if n.shape != (len(self.samplers),):
+            raise ValueError("one sweep count per sampler")
+        cnt = (KvmcCounters * len(self.samplers))()
+        _check(load().kvmc_group_sweep(self._g, _ptr(n), C.cast(cnt, C.c_void_p)))
+        return [c.as_dict() for c in cnt]
+
+    def close(self):
+        if getattr(self, "_g", None):
+            load().kvmc_group_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -38,3 +38,40 @@ def test_autotune_on_device(tmp_path):
     r = autotune(ham, 128, nat.PREC_FP32, 32, steps=1, cache_path=str(tmp_path / "t.json"))
     assert (r["threads"], r["delay"]) in candidates(ham.ns, nat.PREC_FP32)
     assert r["walker_sweeps_per_s"] > 0 and len(r["trials"]) == len(candidates(ham.ns, nat.PREC_FP32))
+
+
+def test_group_graph_matches_per_handle_sweeps():
+    """kvmc_group_sweep (one graph launch over handles of L = 4, 8, 12, real
+    and complex orbitals, FP32 and FP64) leaves every handle exactly where
+    its own kvmc_sweep calls leave a twin handle: configurations, local
+    energies and counters, over calls that change the sweep counts."""
+    from kagome_vmc.batch import flux_cluster
+    specs = [(pi_flux_cluster(4), 64, dict(precision=nat.PREC_FP32, delay=12)),
+             (flux_cluster(8, 0.7), 32, dict(precision=nat.PREC_FP32, delay=12)),
+             (pi_flux_cluster(12), 16, dict(precision=nat.PREC_FP64)),
+             (pi_flux_cluster(12), 16, dict(precision=nat.PREC_FP32, delay=12))]
+    calls = [[24, 96, 216, 216], [24, 96, 216, 216], [10, 0, 216, 50]]
+
+    def make(g, ham, nw, kw):
+        ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, refresh_policy=nat.REFRESH_EVERY_K,
+                               refresh_k=16, seed=40 + g, **kw)
+        ds.init_random(7 + g)
+        return ds
+
+    solo = [make(g, *sp) for g, sp in enumerate(specs)]
+    grouped = [make(g, *sp) for g, sp in enumerate(specs)]
+    try:
+        with nat.SamplerGroup(grouped) as grp:
+            for ns in calls:
+                cg = grp.sweep(ns)
+                for g, (ds1, ds2) in enumerate(zip(solo, grouped)):
+                    c1, _ = ds1.sweep(ns[g])
+                    for key in ("sweeps", "moves", "accepted", "refreshes", "flushes"):
+                        assert c1[key] == cg[g][key], (g, key)
+                    k1, k2 = ds1.get_configs(), ds2.get_configs()
+                    assert np.array_equal(k1[0], k2[0]) and np.array_equal(k1[1], k2[1])
+                    assert np.array_equal(ds1.measure()[0], ds2.measure()[0])
+                assert sum(c["refreshes"] for c in cg) > 0
+    finally:
+        for ds in solo + grouped:
+            ds.close()

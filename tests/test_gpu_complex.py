@@ -196,3 +196,41 @@ def test_complex_refresh_single_kernel_path(oracle_mod, L):
         oc, _, _ = oracle_mod.run_chain(ch, 120, tape_u=tu[w], tape_bond=tb[w])
         np.testing.assert_array_equal(out[w], oc)
         np.testing.assert_array_equal(kf[0][w], ch.kup)
+
+
+def test_mixed_random_flux_groups_follow_oracle(oracle_mod):
+    """The mixed-size batch with a seeded flux phase per lattice (L = 4, 6,
+    8, 10, 12; complex W in FP32 with delay 12, W in LDS for L <= 6 and
+    streamed above; FP64 refresh every 64 accepts, the L = 10 and 12 ones on
+    the single-kernel refresh path): two walkers per group replay in the
+    oracle's complex128 chain (same Philox stream) over two measurement
+    intervals, with the same configurations and local energies."""
+    from kagome_vmc import _native as nat
+    from kagome_vmc.batch import MixedBatch, flux_cluster
+    rng = np.random.default_rng(11)
+    sizes = ((4, 24), (6, 12), (8, 12), (10, 6), (12, 6))
+    groups = [(flux_cluster(L, float(rng.uniform(0.0, np.pi))), nw) for L, nw in sizes]
+    seed, k = 19, 64
+    kw = dict(precision=nat.PREC_FP32, refresh_policy=nat.REFRESH_EVERY_K, refresh_k=k, delay=12, seed=seed)
+    with MixedBatch(groups, **kw) as mb:
+        assert all(g.sampler.complex for g in mb.groups)
+        mb.init_random(3)
+        k0 = [g.sampler.get_configs() for g in mb.groups]
+        r1 = mb.step(1)
+        k1 = [g.sampler.get_configs() for g in mb.groups]
+        r2 = mb.step(1)
+        k2 = [g.sampler.get_configs() for g in mb.groups]
+    diverged = 0
+    for g, (ham, nw) in enumerate(groups):
+        n = min(ham.N_up, ham.N_down)
+        for w in (0, nw - 1):
+            ch = oracle_mod.Chain.start(ham.nn, ham.U_up, ham.U_down, k0[g][0][w], k0[g][1][w])
+            for kk, rr in ((k1, r1), (k2, r2)):
+                oracle_mod.run_chain(ch, n, seed=seed + g, walker=w, policy=1, k=k)
+                if not (np.array_equal(kk[g][0][w], ch.kup) and np.array_equal(kk[g][1][w], ch.kdn)):
+                    diverged += 1
+                    print(f"group L={sizes[g][0]} walker {w}: FP32 chain left the oracle's (near tie)")
+                    break
+                e_or = oracle_mod.getol(ham.nn, ch.kup, ch.kdn, ch.Wup, ch.Wdn)
+                assert abs(rr[g][1][0][w] - e_or) < 1e-4 * max(1.0, abs(e_or)), (g, w)
+    assert diverged <= 1
