@@ -147,6 +147,18 @@ int kvmc_refresh(kvmc_handle* h);
 int kvmc_sweep(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
                int8_t* outcomes, kvmc_counters* counters);
 
+/* No counterpart in the reference.  kvmc_sweep for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid allocation):
+ * the same sweeps, draws, outcomes and counters, with the tape and the
+ * outcome record staged in a fixed 32 MiB device array (one per GPU, held
+ * until the process ends) instead of per-call buffers.  Limit: nwalkers *
+ * nsweeps <= 2581071 when a tape or an outcome record is passed (larger
+ * calls return KVMC_ERR_INVALID); graph replay is not used.  The handle's
+ * device must be current.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (two handles or threads at once) race. */
+int kvmc_sweep_noalloc(kvmc_handle* h, int64_t nsweeps, const double* tape_u,
+                       const int32_t* tape_bond, int8_t* outcomes, kvmc_counters* counters);
+
 /* No counterpart in the reference, whose runs are one lattice each (Carlo
  * farms separate runs).  A group of handles on one GPU -- e.g. lattices of
  * different sizes, one handle each -- whose sweeps run as one graph launch:

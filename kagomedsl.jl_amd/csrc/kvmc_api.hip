@@ -12,11 +12,21 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace kvmc;
 
+// Device scratch of kvmc_sweep_noalloc: the tape (u as double, bond as
+// int32) and the outcome record (int8) of one call, each segment starting
+// on a 256-byte boundary.  Every GPU the library runs on holds its own copy
+// for the life of the process.  In host code `scratch` names a host-side
+// stand-in: its device address comes from hipGetSymbolAddress, once per
+// device (scratch_address below).
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr size_t kScratchMaxEntries = (kScratchBytes - 512) / 13;   // walker-sweeps per call
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -119,6 +129,16 @@ __global__ void k_fill_ull(unsigned long long* p, unsigned long long v, int n) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_i8(signed char* p, signed char v, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Device address of `scratch` on each device, looked up on the first call
+// made with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
 
 // Relative drift of each listed walker (max |W - W_fresh| / max |W_fresh|
 // over both spins, from the per-walker maxima of a drift pass) folded into
@@ -827,9 +847,55 @@ int sweep_staged(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const in
     return sweep_tail(h, a, threads, lds, finished, launches, outcomes, d_out, tape_n, counters);
 }
 
+int scratch_address(const kvmc_handle* h, unsigned char** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != h->device) return fail(KVMC_ERR_INVALID, "the handle's device must be current");
+    if (dev < 0 || dev >= kMaxDevices) return fail(KVMC_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(KVMC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_addr[dev]);
+    return KVMC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int kvmc_sweep_noalloc(kvmc_handle* h, int64_t nsweeps, const double* tape_u, const int32_t* tape_bond,
+                       int8_t* outcomes, kvmc_counters* counters) {
+    g_err.clear();
+    int rc = sweep_check(h, nsweeps, tape_u, tape_bond);
+    if (rc) return rc;
+    if (nsweeps == 0) { if (counters) *counters = kvmc_counters{}; return KVMC_OK; }
+    const size_t tape_n = (size_t)h->nw * (size_t)nsweeps;
+    if ((tape_u || outcomes) && tape_n > kScratchMaxEntries)
+        return fail(KVMC_ERR_INVALID, "nwalkers * nsweeps = " + std::to_string(tape_n) +
+                                          " exceeds the " + std::to_string(kScratchMaxEntries) +
+                                          " entries of the 32 MiB device scratch");
+    if ((rc = tape_check(h, tape_n, tape_u, tape_bond))) return rc;
+    unsigned char* base = nullptr;
+    if ((rc = scratch_address(h, &base))) return rc;
+    auto up256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    double* d_tu = reinterpret_cast<double*>(base);
+    int* d_tb = reinterpret_cast<int*>(base + up256(tape_n * sizeof(double)));
+    signed char* d_out = reinterpret_cast<signed char*>(base + up256(tape_n * sizeof(double)) +
+                                                        up256(tape_n * sizeof(int)));
+    if (outcomes) {
+        // the scratch keeps the previous call's contents: clear the record
+        hipLaunchKernelGGL(k_fill_i8, dim3((unsigned)((tape_n + 255) / 256)), dim3(256), 0, h->stream, d_out,
+                           (signed char)0, (long long)tape_n);
+        HIP_TRY(hipGetLastError());
+    }
+    // graph replay stays off: instantiating a graph may allocate
+    return sweep_staged(h, nsweeps, tape_u, tape_bond, outcomes, counters, d_tu, d_tb, d_out, false);
+}
 
 int kvmc_group_create(kvmc_handle* const* handles, int32_t n, kvmc_group** out) {
     g_err.clear();

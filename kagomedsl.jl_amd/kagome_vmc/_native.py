@@ -26,7 +26,7 @@ EXPORTS = [
     "kvmc_measure_grad", "kvmc_get_logdet", "kvmc_get_w", "kvmc_set_sweep_count",
     "kvmc_get_sweep_count", "kvmc_w_drift", "kvmc_failed_walker", "kvmc_last_error",
     "kvmc_version", "kvmc_profile_enable", "kvmc_get_profile", "kvmc_stagger_refresh",
-    "kvmc_enable_graph", "kvmc_track_drift", "kvmc_get_drift_stats",
+    "kvmc_enable_graph", "kvmc_track_drift", "kvmc_get_drift_stats", "kvmc_sweep_noalloc",
     "kvmc_group_create", "kvmc_group_sweep", "kvmc_group_destroy",
 ]
 
@@ -91,6 +91,7 @@ def load(path: str = LIB_PATH):
         "kvmc_init_random": ([vp, C.c_uint64, C.c_int32, P(C.c_int64)], C.c_int),
         "kvmc_refresh": ([vp], C.c_int),
         "kvmc_sweep": ([vp, C.c_int64, vp, vp, vp, P(KvmcCounters)], C.c_int),
+        "kvmc_sweep_noalloc": ([vp, C.c_int64, vp, vp, vp, P(KvmcCounters)], C.c_int),
         "kvmc_group_create": ([vp, C.c_int32, P(vp)], C.c_int),
         "kvmc_group_sweep": ([vp, vp, vp], C.c_int),
         "kvmc_group_destroy": ([vp], C.c_int),
@@ -210,16 +211,18 @@ class DeviceSampler:
         _check(load().kvmc_refresh(self._h))
 
     # --- hot path -------------------------------------------------------------
-    def sweep(self, nsweeps, tape_u=None, tape_bond=None, record=False):
-        """Runs nsweeps sweeps on every walker; returns (counters, outcomes|None)."""
+    def sweep(self, nsweeps, tape_u=None, tape_bond=None, record=False, noalloc=False):
+        """Runs nsweeps sweeps on every walker; returns (counters, outcomes|None).
+        noalloc: stage the tape and outcomes in the library's fixed device
+        scratch instead of per-call buffers (kvmc_sweep_noalloc)."""
         cnt = KvmcCounters()
         tu = tb = None
         if tape_u is not None:
             tu = np.ascontiguousarray(tape_u, dtype=np.float64).reshape(self.nwalkers, nsweeps)
             tb = np.ascontiguousarray(tape_bond, dtype=np.int32).reshape(self.nwalkers, nsweeps)
         out = np.zeros((self.nwalkers, nsweeps), dtype=np.int8) if record else None
-        _check(load().kvmc_sweep(self._h, int(nsweeps), _ptr(tu), _ptr(tb), _ptr(out),
-                                 C.byref(cnt)))
+        fn = load().kvmc_sweep_noalloc if noalloc else load().kvmc_sweep
+        _check(fn(self._h, int(nsweeps), _ptr(tu), _ptr(tb), _ptr(out), C.byref(cnt)))
         return cnt.as_dict(), out
 
     def measure(self):
