@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "kagome_vmc", "libkagome_vmc.so")
-SOURCES = ["kvmc_device.hip", "kvmc_complex.hip", "kvmc_refresh.hip", "kvmc_api.hip"]
+SOURCES = ["kvmc_device.hip", "kvmc_sweep_ib.hip", "kvmc_complex.hip", "kvmc_refresh.hip", "kvmc_api.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off: plain v_fma_f32 only

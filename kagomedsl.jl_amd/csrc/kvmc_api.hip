@@ -201,6 +201,7 @@ struct kvmc_handle {
     void* A = nullptr;
     void* Bm = nullptr;
     int* npend = nullptr;
+    int* pterm = nullptr;           // [nw][2][kMaxDelay][2] rows and labels of the pending terms
     int* zmu = nullptr;
     int* acc_since = nullptr;
     int* status = nullptr;
@@ -248,7 +249,7 @@ namespace {
 void free_all(kvmc_handle* h) {
     void* ptrs[] = {h->bonds, h->nbr_off, h->nbr, h->Uup, h->Udn, h->UTup, h->UTdn, h->Ucup, h->Ucdn, h->kRup,
                     h->kRdn, h->eloc_im, h->kup, h->kdn, h->W, h->slot, h->A,
-                    h->Bm, h->npend, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
+                    h->Bm, h->npend, h->pterm, h->zmu, h->acc_since, h->status, h->sweeps, h->stop,
                     h->tbase, h->rlist, h->rcount, h->failed, h->all_walkers, h->counters,
                     h->logdet, h->eloc, h->drift, h->dstats, h->ws, h->Qup, h->Qdn, h->O};
     for (void* p : ptrs)
@@ -281,6 +282,7 @@ SweepArgs sweep_args(kvmc_handle* h) {
     a.bonds = h->bonds; a.nbr_off = h->nbr_off; a.nbr = h->nbr;
     a.kup = h->kup; a.kdn = h->kdn; a.W = h->W; a.slot = h->slot; a.A = h->A; a.Bm = h->Bm;
     a.npend = h->npend; a.delay = h->delay; a.nmax = h->nmax;
+    a.pterm = h->pterm;
     a.sweeps = h->sweeps; a.stop = h->stop; a.tape_base = h->tbase;
     a.zmu = h->zmu; a.acc_since = h->acc_since;
     a.refresh_list = h->rlist; a.refresh_count = h->rcount;
@@ -507,7 +509,7 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
         ALLOC(p, nw * 2 * pend_ld(delay) * (size_t)w_nsa(h->ns, h->nup, h->ndn) * h->esize); h->A = p;
         ALLOC(p, nw * 2 * pend_ld(delay) * h->nmax * h->esize); h->Bm = p;
     }
-    ALLOC(h->npend, nw); ALLOC(h->zmu, nw); ALLOC(h->acc_since, nw); ALLOC(h->status, nw);
+    ALLOC(h->npend, nw); ALLOC(h->pterm, nw * 2 * kMaxDelay * 2); ALLOC(h->zmu, nw); ALLOC(h->acc_since, nw); ALLOC(h->status, nw);
     ALLOC(h->sweeps, nw); ALLOC(h->stop, nw); ALLOC(h->tbase, nw);
     ALLOC(h->rlist, nw); ALLOC(h->rcount, 1); ALLOC(h->failed, 1); ALLOC(h->all_walkers, nw);
     ALLOC(h->counters, (size_t)CNT_SLOTS);

@@ -17,6 +17,9 @@ size_t sweep_lds_bytes(size_t esize, int ns, int nup, int ndn, int delay, int nb
 hipError_t launch_sweep(int prec, const SweepArgs& a, int nw, int threads, size_t lds, hipStream_t s) {
     // rows of W are padded to a multiple of 4: always whole 16-byte vectors
     if (prec == 0) return launch_sweep_v<double, 2>(a, nw, threads, lds, s);
+    // FP32 W streamed from HBM with columns of at most 256 rows: pending rows
+    // kept implicit (no row reads at an accept)
+    if (sweep_ib_supported(prec, 0, a.ns, a.nup, a.ndn, a.delay)) return launch_sweep_ib(a, nw, s);
     return launch_sweep_v<float, 4>(a, nw, threads, lds, s);
 }
 

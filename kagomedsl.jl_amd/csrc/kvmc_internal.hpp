@@ -134,6 +134,9 @@ struct SweepArgs {
     signed char* outcomes;          // [nw][tape_len] or null
     int kernel_mode, policy, refresh_k;
     int walker0;                    // first walker of this launch
+    // FP32 sweep with implicit pending rows (k_sweep_ib): the row slot K_q
+    // and label l_q of each pending term, [nw][2][kMaxDelay][2]
+    int* pterm;
 };
 
 struct RefreshArgs {
@@ -205,6 +208,10 @@ hipError_t launch_sweep(int prec, const SweepArgs& a, int nwalkers, int threads,
                         hipStream_t s);
 hipError_t launch_sweep_c(int prec, const SweepArgs& a, int nwalkers, int threads, size_t lds,
                           hipStream_t s);
+// FP32 sweep with implicit pending rows (kvmc_sweep_ib.hip): real W streamed
+// from HBM, at most 256 rows per spin, 1 < delay <= kMaxDelay.
+bool sweep_ib_supported(int prec, int cplx, int ns, int nup, int ndn, int delay);
+hipError_t launch_sweep_ib(const SweepArgs& a, int nwalkers, hipStream_t s);
 size_t sweep_lds_bytes(size_t esize, int ns, int nup, int ndn, int delay, int nb);
 // Both W of a walker at most this large: the sweep kernel keeps them in LDS.
 constexpr size_t kResidentMaxBytes = 64 * 1024;

@@ -62,11 +62,20 @@ __device__ __forceinline__ double abs2d(double x) { return x * x; }
 __device__ __forceinline__ double abs2d(cf32 x) { return (double)x.re * (double)x.re + (double)x.im * (double)x.im; }
 __device__ __forceinline__ double abs2d(cf64 x) { return x.re * x.re + x.im * x.im; }
 
-// Wavefront shuffle of a W element.
-__device__ __forceinline__ float shfl_t(float v, int l) { return __shfl(v, l, 64); }
-__device__ __forceinline__ double shfl_t(double v, int l) { return __shfl(v, l, 64); }
-__device__ __forceinline__ cf32 shfl_t(cf32 v, int l) { return cf32(__shfl(v.re, l, 64), __shfl(v.im, l, 64)); }
-__device__ __forceinline__ cf64 shfl_t(cf64 v, int l) { return cf64(__shfl(v.re, l, 64), __shfl(v.im, l, 64)); }
+// Value of lane l (uniform l) as a scalar, for a W element of any storage type.
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_t(float v, int l) { return readlane_f(v, l); }
+__device__ __forceinline__ double lane_t(double v, int l) { return readlane_d(v, l); }
+__device__ __forceinline__ cf32 lane_t(cf32 v, int l) { return cf32(readlane_f(v.re, l), readlane_f(v.im, l)); }
+__device__ __forceinline__ cf64 lane_t(cf64 v, int l) { return cf64(readlane_d(v.re, l), readlane_d(v.im, l)); }
 
 // FP64 accumulator of a W element: double, or cf64 for complex W.
 template <typename T> struct AccT { using type = double; };
@@ -431,16 +440,18 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k
         n_sweeps += ncommit;
         if (!tmask) continue;
         const int tl = ncommit - 1;   // lane of the terminal sweep
-        const bool t_acc = __shfl((int)acc, tl, 64) != 0;
+        // readlane (not a shuffle) keeps the terminal sweep's values scalar,
+        // so the term count and everything branching on it stay uniform
+        const bool t_acc = __builtin_amdgcn_readlane((int)acc, tl) != 0;
         const long long t_sw = sweeps - 1;
         bool refresh_due;
         if (t_acc) {
-            const int ti = __shfl(i, tl, 64), tsite = __shfl(site, tl, 64);
-            const bool tcan1 = __shfl((int)can1, tl, 64) != 0;
-            const int tlup = __shfl(lup, tl, 64), tldn = __shfl(ldn, tl, 64);
-            const int tKup = __shfl(Kup, tl, 64), tKdn = __shfl(Kdn, tl, 64);
-            const T twu = shfl_t(wu, tl), twd = shfl_t(wd, tl);
-            const int tdz = __shfl(dz, tl, 64);
+            const int ti = __builtin_amdgcn_readlane(i, tl), tsite = __builtin_amdgcn_readlane(site, tl);
+            const bool tcan1 = __builtin_amdgcn_readlane((int)can1, tl) != 0;
+            const int tlup = __builtin_amdgcn_readlane(lup, tl), tldn = __builtin_amdgcn_readlane(ldn, tl);
+            const int tKup = __builtin_amdgcn_readlane(Kup, tl), tKdn = __builtin_amdgcn_readlane(Kdn, tl);
+            const T twu = lane_t(wu, tl), twd = lane_t(wd, tl);
+            const int tdz = __builtin_amdgcn_readlane(dz, tl);
             ++n_acc;
             ++acc_since;
             const int eu = max(s_slu[tKup], 0), ed = max(s_sld[tKdn], 0);   // rows of the target sites
