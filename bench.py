@@ -48,9 +48,10 @@ FP64_PEAK_TFLOPS = 78.6
 # walker refreshed by the kernel pair k_refresh_blocked + k_refresh_solve, from
 # rocprofv3 --pmc passes over this script's default FP32 run (summaries in
 # profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_155_310}
-# the same for k_sweep, per walker-sweep (FP32, delay 12, L=12)
-SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 18_014}
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 7_309_431}
+# the same for the FP32 sweep kernel (k_sweep_ib), per walker-sweep (delay 12,
+# L=12, refresh every 216 accepts)
+SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 9_897}
 
 # closed-shell boundary conditions per cluster size
 BC = {4: ((True, True), (False, True)), 6: ((True, True), (False, False)),
@@ -342,7 +343,18 @@ def roofline_refresh(r, ns, traffic_per_walker):
             "units_per_launch": wpl, "launch_ms": r["refresh_ms_per_launch"]}
 
 
-def roofline_sweep(r, traffic_per_walker_sweep, ham, s=4):
+def sweep_kernel_name(ham, s, delay):
+    """The sweep kernel the library runs for this shape: FP32 W streamed from
+    HBM with at most 256 empty-site rows per spin and 1 < delay <= 16 uses
+    the implicit-pending-row kernel, everything else k_sweep."""
+    rows = [((ham.ns - n) + 3) // 4 * 4 for n in (ham.N_up, ham.N_down)]
+    resident = s * sum(r * n for r, n in zip(rows, (ham.N_up, ham.N_down))) <= 64 * 1024
+    if s == 4 and 1 < delay <= 16 and max(rows) <= 256 and not resident:
+        return "k_sweep_ib (exchange step + delayed Sherman-Morrison updates, pending rows implicit)"
+    return "k_sweep (exchange step + delayed Sherman-Morrison updates)"
+
+
+def roofline_sweep(r, traffic_per_walker_sweep, ham, s=4, delay=12):
     """HBM roofline of the sweep kernel.  `achieved` counts the bytes the
     algorithm as built must move (flushes of the empty-site rows of W once
     per `delay` accepts, the column and row of each accept, two gathers per
@@ -360,7 +372,7 @@ def roofline_sweep(r, traffic_per_walker_sweep, ham, s=4):
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_per_walker_sweep * wsl if traffic_per_walker_sweep else None,
             "units_per_launch": wsl,
-            "kernel": "k_sweep (exchange step + delayed Sherman-Morrison updates)",
+            "kernel": sweep_kernel_name(ham, s, delay),
             "launch_ms": r["sweep_ms_per_launch"],
             "achieved_per_accept_unit": per_accept,
             "frac_per_accept_unit": per_accept / HBM_PEAK_GBS}
@@ -611,7 +623,8 @@ def main():
     # the dominant kernel of the headline form carries the roofline
     traffic_ok = args.delay == 12 and args.policy == "every_k" and args.k == 216
     rr = roofline_refresh(r32, ham.ns, REFRESH_TRAFFIC_BYTES_PER_WALKER.get(L) if traffic_ok else None)
-    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if traffic_ok else None, ham)
+    rs = roofline_sweep(r32, SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP.get(L) if traffic_ok else None, ham,
+                        delay=args.delay)
     dominant = rr if rr and r32["kernel_ms"]["refresh_ms"] > r32["kernel_ms"]["sweep_ms"] else rs
     value = r32["walker_sweeps_per_s"]
     line = {
