@@ -180,6 +180,13 @@ __device__ __forceinline__ unsigned long long pivot_key(double v, int row) {
     return (b & ~0xFFFull) | (unsigned long long)(4095 - row);
 }
 
+// 1/x to about one ulp: hardware reciprocal and two Newton steps.
+__device__ __forceinline__ double rcp_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 template <typename T, int RPT, bool FULL>
 __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : LU_WAVES))) k_refresh_blocked(RefreshArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -341,7 +348,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                         const int tmp = phys[kk]; phys[kk] = phys[p]; phys[p] = tmp;
                         pivabs[kk] = fabs(piv);
                     }
-                    const double rp = 1.0 / piv;
+                    const double rp = rcp_f64(piv);
 #pragma unroll
                     for (int q = 0; q < RPT; ++q) {
                         const int r = myr[q];
