@@ -39,11 +39,10 @@ __host__ __device__ inline size_t sweep_ib_lds(int ns, int nsa, int delay, int n
     const size_t nsp = (size_t)(ns + 3) / 4 * 4;
     const size_t b = 4 * nsp * sizeof(int16_t)                  // kappa and slot maps
                      + 2 * (size_t)delay * nsa * sizeof(float)    // At columns
-                     + 2 * (size_t)nsa * sizeof(float)            // the accept's columns
                      + 16 * sizeof(double)                        // reductions
                      + 4 * (size_t)delay * sizeof(int)            // term rows and labels
                      + 2 * (size_t)delay * sizeof(float)          // an accept's coefficients
-                     + 4 * (size_t)(delay <= 12 ? 4 : 2) * 256 * sizeof(float)   // flush staging, 4 wavefronts
+                     + 4 * (size_t)2 * 256 * sizeof(float)   // flush staging, 4 wavefronts (also the accept's columns)
                      + (size_t)nb * sizeof(uint32_t) + (size_t)(ns + 1 + 2 * nb) * sizeof(uint16_t);
     return (b + 15) / 16 * 16;
 }
@@ -105,11 +104,15 @@ __device__ __forceinline__ void flush_ib(float* __restrict__ W0, const float* __
     }
 }
 
-// columns of W in flight per wavefront in the flush
-template <int DMAX> struct FlushUnroll { static constexpr int value = DMAX <= 12 ? 4 : 2; };
+// Columns of W in flight per wavefront in the flush.  Two keep the LDS of a
+// walker at L=12 and delay 12 within 40 KB, so four walkers share a CU.
+template <int DMAX> struct FlushUnroll { static constexpr int value = 2; };
 
+// Four waves per SIMD (128 VGPRs) up to delay 12: four walkers per CU hide
+// more of the proposal and accept latency (8.8 % less sweep time at L=12 than
+// three); the delay-16 register set needs three.
 template <int DMAX, int WIN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_sweep_ib(SweepArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <= 12 ? 4 : 3))) k_sweep_ib(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int w = a.walker0 + (int)blockIdx.x;
     const int ns = a.ns, nup = a.nup, ndn = a.ndn, nb = a.nb, nmax = a.nmax;
@@ -121,9 +124,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     int16_t* s_slu = s_kdn + nsp;                    // site -> row of W_up (-1: occupied)
     int16_t* s_sld = s_slu + nsp;
     float* s_At = reinterpret_cast<float*>(s_sld + nsp);   // [2][delay][nsa]
-    float* s_col = s_At + 2 * delay * nsa;                 // [2][nsa]
     constexpr int UNR = FlushUnroll<DMAX>::value;
-    float* s_stage = s_col + 2 * nsa;                      // [4 wavefronts][UNR][256] flush staging (16-byte aligned)
+    float* s_stage = s_At + 2 * delay * nsa;               // [4 wavefronts][UNR][256] flush staging (16-byte aligned)
+    float* s_col = s_stage;                                // [2][nsa] an accept's columns (not live during a flush)
     double* s_red = reinterpret_cast<double*>(s_stage + 4 * UNR * 256);
     int* s_K = reinterpret_cast<int*>(s_red + 16);         // [2][delay] term rows
     int* s_L = s_K + 2 * delay;                            // [2][delay] term labels
@@ -235,7 +238,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                 const float* cu = Wup + (size_t)lup * ldu;
                 const float* cd = Wdn + (size_t)ldn * ldd;
                 // every gather is issued before any is used (one memory round
-                // trip): slots past the live terms re-read row e, in bounds
+                // trip): slots past the live terms re-read row e, in bounds.
+                // (Skipping the groups of 4 past c with a uniform branch kept
+                // more registers live: 35 spills instead of 12, 11 % slower.)
                 float gu[DMAX], gd[DMAX];
 #pragma unroll
                 for (int q = 0; q < DMAX; ++q) {
