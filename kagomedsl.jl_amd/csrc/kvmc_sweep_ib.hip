@@ -293,22 +293,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <
             ++n_acc;
             ++acc_since;
             const int eu = max((int)s_slu[tKup], 0), ed = max((int)s_sld[tKdn], 0);
-            // the columns of the moving labels, as stored
-            // (one element per thread and spin: blockDim.x >= nsa)
+            const float au = -1.f / twu, ad = -1.f / twd;
+            // the columns of the moving labels, as stored (one element per
+            // thread and spin: blockDim.x >= nsa), and the new term's
+            // coefficients on the old terms, alpha At_q[K].  At is unchanged
+            // until the barrier: every wavefront has left its window.
             {
                 const float cu = tid < ldu ? Wup[(size_t)tlup * ldu + tid] : 0.f;
                 const float cd = tid < ldd ? Wdn[(size_t)tldn * ldd + tid] : 0.f;
+                if (tid < c) {
+                    s_tc[tid] = au * sAu[tid * nsa + eu];
+                    s_tc[delay + tid] = ad * sAd[tid * nsa + ed];
+                }
                 if (tid < nsa) { s_col[tid] = cu; s_col[nsa + tid] = cd; }
             }
             __syncthreads();
-            // A_c = W_cur[:, l] + e_K (one row per thread and spin), and the new
-            // term's coefficients on the old terms, alpha At_q[K], saved before
-            // any At changes
-            const float au = -1.f / twu, ad = -1.f / twd;
-            if (tid < c) {
-                s_tc[tid] = au * sAu[tid * nsa + eu];
-                s_tc[delay + tid] = ad * sAd[tid * nsa + ed];
-            }
+            // A_c = W_cur[:, l] + e_K, one row per thread and spin; each thread
+            // then updates only its own row of At, so no barrier is needed
+            // between reading and writing At
             float Au = 0.f, Ad = 0.f;
             if (tid < ldu) {
                 float v = s_col[tid];
@@ -322,7 +324,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <
                     v += sAd[q * nsa + tid] * (s_col[nsa + s_K[delay + q]] - (tldn == s_L[delay + q] ? 1.f : 0.f));
                 Ad = v + (tid == ed ? 1.f : 0.f);
             }
-            __syncthreads();
             if (tid < nsa) {
                 const bool ru = tid < ldu, rd = tid < ldd;
                 for (int q = 0; q < c; ++q) {
