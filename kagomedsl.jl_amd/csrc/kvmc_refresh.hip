@@ -197,9 +197,16 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     const int spin = job & 1;
     const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
     const int nrows = refresh_nrows(ns), ldm = refresh_ldm(nmax);
-    const int nr16 = rup_i(ns, 16);      // logical rows that carry data
+    // Empty-site rows the single kernel carries: all of them, or, for the
+    // real form of complex orbitals (2 ns_out sites, site ns_out + s a copy
+    // of site s), those of the first ns_out sites only, the rows W is made
+    // of.  The pivots come from the rows of tildeU, so the others are not
+    // needed.
+    const int nkeep = (FULL && a.cplx_out) ? a.ns_out : ns;
+    const int nE = (FULL && a.cplx_out) ? a.ns_out - (spin ? a.ndn_out : a.nup_out) : ns - N;
+    const int nr16 = rup_i(N + nE, 16);  // logical rows that carry data
     const int nc16 = rup_i(N, 16);       // columns that carry data
-    const int m16 = rup_i(ns - N, 16);   // empty-site rows, padded
+    const int m16 = rup_i(nE, 16);       // empty-site rows, padded
     const double* __restrict__ UT = spin ? a.UTdn : a.UTup;   // ns x N row-major
     const int* __restrict__ kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
     double* __restrict__ M = a.ws + (size_t)job * refresh_ws_job(ns, nmax);
@@ -238,7 +245,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     for (int s0 = 0; s0 < ns; s0 += RT) {
         const int s = s0 + t;
         const int l = s < ns ? kap[s] : -1;
-        const bool empty = (l == 0);
+        const bool empty = (l == 0) && s < nkeep;
         const unsigned long long m = __ballot(empty);
         const int before = __popcll(m & ((1ull << lane) - 1ull));
         if (lane == 0) wsum[wave] = __popcll(m);
@@ -250,8 +257,10 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         if (s < ns) {
             if (empty) {
                 const int row = N + off + before;
-                if (row < ns) { rowsite[row] = s; rowof[s] = row; }
+                if (row < N + nE) { rowsite[row] = s; rowof[s] = row; }
                 else s_bad = 1;
+            } else if (l == 0) {
+                // an empty site of the copied half (real form): no row
             } else if (l < 1 || l > N) {
                 s_bad = 1;
             } else {
@@ -263,7 +272,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         base += tot;
         __syncthreads();
     }
-    if (t == 0 && base != ns - N) s_bad = 1;
+    if (t == 0 && base != nE) s_bad = 1;
     __syncthreads();
     if (s_bad) {
         if (t == 0) { a.status[w] = 2; atomicMin(a.failed, w); }
@@ -278,7 +287,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     // operands straight from U (L2-resident, shared by every job) and every
     // later read of M finds values written by an earlier step.
     auto u_at = [&](int prow, int col) -> double {
-        return (prow < (FULL ? ns : N) && col < N) ? UT[(size_t)rowsite[prow] * N + col] : 0.0;
+        return (prow < (FULL ? N + nE : N) && col < N) ? UT[(size_t)rowsite[prow] * N + col] : 0.0;
     };
 
     // ---- 1. blocked LU ---------------------------------------------------
