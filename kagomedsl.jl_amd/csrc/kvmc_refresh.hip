@@ -47,6 +47,7 @@ constexpr int RT = 256;   // threads per workgroup
 constexpr int NWAVE = RT / 64;
 constexpr int LB = RB + 1;  // LDS row stride of 32-wide blocks
 constexpr int LU_WAVES = 3;  // waves per SIMD of the LU-only kernel
+constexpr int kStoreBlock = 8;   // sites per LDS block of the single kernel's W store
 
 __host__ __device__ inline int rup_i(int x, int m) { return (x + m - 1) / m * m; }
 __host__ __device__ inline int refresh_nrows(int ns) { return rup_i(ns, 16) + 16; }
@@ -224,9 +225,8 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         reinterpret_cast<unsigned long long*>(cand + 2 * NWAVE * RB);   // [2][NWAVE]
     double* Dblk = reinterpret_cast<double*>(candk + 2 * NWAVE);    // [RB][LB] factored diagonal block
     double* Linv = Dblk + RB * LB;                          // [RB][LB] U11^-1, then L11^-1
-    const int su = max(nc16 - RB, 16);                      // U12 row stride in LDS
-    double* stage = Linv + RB * LB;                         // U12 [RB][su] / scatter block
-    const int stage_len = FULL ? max(RB * su, 16 * (nc16 | 1)) : 0;   // U12 / W staging (single kernel)
+    double* stage = Linv + RB * LB;                         // W scatter block (single kernel)
+    const int stage_len = FULL ? kStoreBlock * (nc16 | 1) : 0;
     double* pivabs = stage + stage_len;                     // [nmax] |pivot|
     double* redd = pivabs + nmax;                           // [3 * NWAVE]
     int* wsum = reinterpret_cast<int*>(redd + 3 * NWAVE);   // [NWAVE]
@@ -451,10 +451,6 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 const int r0 = (lane >> 4) + 4 * g, r1 = r0 + 16;
                 M[(size_t)phys[j0 + r0] * ldm + c0 + cc] = u0[g];
                 M[(size_t)phys[j0 + r1] * ldm + c0 + cc] = u1[g];
-                if (FULL) {
-                    stage[r0 * su + cc] = u0[g];
-                    stage[r1 * su + cc] = u1[g];
-                }
             }
         }
         __syncthreads();
@@ -473,12 +469,11 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 prow[g] = phys[r0 + 16 * ti + (lane >> 4) + 4 * g];
                 crow[g] = M + (size_t)prow[g] * ldm + c0 + (lane & 15);
             }
-            // U12 operand rows: LDS (single-kernel path) or M itself (L2)
+            // U12 operand rows, read back from M through L2 (staged in LDS
+            // they took 102 KB at N = 432, one workgroup per CU)
             const double* urow[RB / 4];
 #pragma unroll
-            for (int s = 0; s < RB / 4; ++s)
-                urow[s] = FULL ? stage + (4 * s + (lane >> 4)) * su
-                               : M + (size_t)phys[j0 + 4 * s + (lane >> 4)] * ldm + c0;
+            for (int s = 0; s < RB / 4; ++s) urow[s] = M + (size_t)phys[j0 + 4 * s + (lane >> 4)] * ldm + c0;
             // C operands: from U in the first panel, from M afterwards
             auto cval = [&](int g, int tj) -> double {
                 return j0 == 0 ? u_at(prow[g], c0 + 16 * tj + (lane & 15)) : crow[g][16 * tj];
@@ -627,7 +622,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     // sites are staged in LDS (contiguous row reads, odd row stride against
     // bank conflicts), then every label column of W gets the block's entries
     // written as one contiguous run.
-    constexpr int SB = 16;
+    constexpr int SB = kStoreBlock;
     const int xs = (nc16 | 1);                       // LDS row stride (odd)
     double* xb = stage;                              // [SB][xs]
     const int nsa = w_nsa(nso, a.nup_out, a.ndn_out);
@@ -947,10 +942,7 @@ size_t refresh_ws_doubles_per_job(int ns, int nmax) { return refresh_ws_job(ns, 
 
 static size_t refresh_lds_bytes(int ns, int nup, int ndn, bool full = true) {
     const int nmax = nup > ndn ? nup : ndn;
-    const int nmin = nup < ndn ? nup : ndn;
-    const int su = rup_i(nmax, 16) - RB > 16 ? rup_i(nmax, 16) - RB : 16;
-    const int stage = !full ? 0 : RB * su > 16 * (rup_i(nmax, 16) | 1) ? RB * su : 16 * (rup_i(nmax, 16) | 1);
-    (void)nmin;
+    const int stage = !full ? 0 : kStoreBlock * (rup_i(nmax, 16) | 1);
     const size_t dbl = (size_t)(2 * NWAVE * RB + 2 * NWAVE + 2 * RB * LB + stage + nmax + 3 * NWAVE) * sizeof(double);
     const size_t ints = (size_t)(NWAVE + refresh_nrows(ns) + 2 * ns + nmax) * sizeof(int);
     return dbl + ints + 16;

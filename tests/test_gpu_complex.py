@@ -20,17 +20,18 @@ PHI, B = np.pi / 3, 0.1
 _CACHE = {}
 
 
-def complex_system(L, phi=PHI, field=B):
-    key = (L, phi, field)
+def complex_system(L, phi=PHI, field=B, fillings=None):
+    key = (L, phi, field, fillings)
     if key not in _CACHE:
         from kagome_vmc import DoubleKagome, Hamiltonian
         from kagome_vmc.hamiltonian import flux_link_tables
         pbc, apbc = BC[L]
         lat = DoubleKagome.build(1.0, L, L, pbc, apbc)
         n = 3 * L * L // 2
+        nu, nd = fillings or (n, n)
         tin, tint = flux_link_tables(lat, phi)
         # the J1 list of the pi-flux geometry (same nonzero pattern)
-        _CACHE[key] = Hamiltonian.build(n, n, lat, tin, tint, B=field)
+        _CACHE[key] = Hamiltonian.build(nu, nd, lat, tin, tint, B=field)
     return _CACHE[key]
 
 
@@ -171,12 +172,14 @@ def test_complex_fp32_tape_parity(oracle_mod, L, delay):
     assert c2["flushes"] > 0 and int((out == 3).sum()) > 0
 
 
-@pytest.mark.parametrize("L", [10, 12])
-def test_complex_refresh_single_kernel_path(oracle_mod, L):
+@pytest.mark.parametrize("L,fillings", [(10, None), (12, None), (10, (152, 148))])
+def test_complex_refresh_single_kernel_path(oracle_mod, L, fillings):
     """At L = 10 and 12 the real form has 2N = 300 and 432 columns, beyond the
-    register-tile solve: the single-kernel refresh path factors it.  W and
-    log|det| against the oracle; then a short FP64 tape chain."""
-    ham = complex_system(L)
+    register-tile solve: the single-kernel refresh path factors it (keeping
+    the empty-site rows of the first ns sites only), also with unequal
+    fillings.  W and log|det| against the oracle; then a short FP64 tape
+    chain."""
+    ham = complex_system(L, fillings=fillings)
     nw = 2
     ku, kd = conditioned(oracle_mod, ham, nw, 5)
     tu, tb = tape(nw, 120, len(ham.nn), 9)
