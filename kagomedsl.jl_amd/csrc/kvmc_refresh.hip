@@ -328,9 +328,11 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 for (int q = 0; q < RPT; ++q) {
                     if (bq == q && key == wkey && wkey != 0ull) {
 #pragma unroll
-                        for (int c = jj & ~1; c < RB; c += 2)
-                            *reinterpret_cast<double2*>(&cand[(par * NWAVE + wave) * RB + c]) =
-                                make_double2(x[q][c], x[q][c + 1]);
+                        for (int c = 0; c < RB; c += 2) {
+                            if (c >= (jj & ~1))
+                                *reinterpret_cast<double2*>(&cand[(par * NWAVE + wave) * RB + c]) =
+                                    make_double2(x[q][c], x[q][c + 1]);
+                        }
                     }
                 }
                 if (lane == 0) candk[par * NWAVE + wave] = wkey;
@@ -365,8 +367,11 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                         if (myr[q] > kk && myp[q] >= 0) {
                             const double l = x[q][jj] * rp;
                             x[q][jj] = l;
+                            // constant bounds, so that x stays in registers
+                            // once the jj loop is unrolled
 #pragma unroll
-                            for (int c = jj + 1; c < RB; ++c) x[q][c] -= l * prow[c];
+                            for (int c = 1; c < RB; ++c)
+                                if (c > jj) x[q][c] -= l * prow[c];
                         }
                     }
                 }
@@ -1014,6 +1019,7 @@ static hipError_t launch_refresh_t(const RefreshArgs& a, size_t lds, hipStream_t
     switch (refresh_rows_per_thread(a.nmax)) {
         case 1: return launch_lu<T, 1, true>(a, lds, s);
         case 2: return launch_lu<T, 2, true>(a, lds, s);
+        case 3: return launch_lu<T, 3, true>(a, lds, s);
         default: return launch_lu<T, 4, true>(a, lds, s);
     }
 }

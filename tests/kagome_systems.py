@@ -6,10 +6,11 @@ from kagome_vmc import DoubleKagome, Hamiltonian
 
 # Boundary conditions that close the shell at half filling (checked by
 # test_host_setup.test_shell_closure): antiperiodic along a2 for L = 4, 8, 12
-# (as the reference's scripts/job.jl:17-18), plain periodic for L = 6, 10.
+# (as the reference's scripts/job.jl:17-18), plain periodic for L = 6, 10, 14.
 BC = {2: ((True, True), (False, True)), 4: ((True, True), (False, True)),
       6: ((True, True), (False, False)), 8: ((True, True), (False, True)),
-      10: ((True, True), (False, False)), 12: ((True, True), (False, True))}
+      10: ((True, True), (False, False)), 12: ((True, True), (False, True)),
+      14: ((True, True), (False, False))}
 
 _CACHE = {}
 

@@ -172,11 +172,12 @@ def test_complex_fp32_tape_parity(oracle_mod, L, delay):
     assert c2["flushes"] > 0 and int((out == 3).sum()) > 0
 
 
-@pytest.mark.parametrize("L,fillings", [(10, None), (12, None), (10, (152, 148))])
+@pytest.mark.parametrize("L,fillings", [(10, None), (12, None), (14, None), (10, (152, 148))])
 def test_complex_refresh_single_kernel_path(oracle_mod, L, fillings):
-    """At L = 10 and 12 the real form has 2N = 300 and 432 columns, beyond the
-    register-tile solve: the single-kernel refresh path factors it (keeping
-    the empty-site rows of the first ns sites only), also with unequal
+    """At L = 10, 12 and 14 the real form has 2N = 300, 432 and 588 columns,
+    beyond the register-tile solve: the single-kernel refresh path factors it
+    (keeping the empty-site rows of the first ns sites only; two panel rows
+    per thread at L = 10 and 12, three at L = 14), also with unequal
     fillings.  W and log|det| against the oracle; then a short FP64 tape
     chain."""
     ham = complex_system(L, fillings=fillings)

@@ -51,10 +51,11 @@ def test_hmat_matches_oracle(oracle_mod, n1, n2, pbc, apbc, B):
 
 @pytest.mark.parametrize("L,apbc,closed", [(4, (False, True), True), (4, (False, False), False),
                                            (6, (False, False), True), (8, (False, True), True),
-                                           (12, (False, True), True)])
+                                           (12, (False, True), True), (14, (False, False), True),
+                                           (14, (False, True), False)])
 def test_shell_closure(L, apbc, closed):
     """Closed shells (read off the spectrum) decide the boundary conditions:
-    antiperiodic along a2 for L = 4, 8, 12, plain periodic for L = 6."""
+    antiperiodic along a2 for L = 4, 8, 12, plain periodic for L = 6, 14."""
     H = Hmat(DoubleKagome.build(1.0, L, L, (True, True), apbc))
     _, _, w = orbitals(H, 3 * L * L // 2, 3 * L * L // 2)
     assert (shell_gap(w, 3 * L * L // 2) > 1e-6) == closed
