@@ -681,32 +681,34 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Stages row block J of the factors (logical rows 16J..16J+15 of M, NT * 16
-// columns; row stride LDBK in LDS) into buf and the 16x16 diagonal block of
-// the saved 32x32 inverse `inv` into dbuf with direct global-to-LDS loads
-// (global_load_lds_dwordx4: lane l's 16 bytes land at dst + 16 l).  buf is
-// 2 (NT + 1) chunks of 1 KiB and dbuf 2, dealt to the wavefronts in turn;
-// the LDS row padding receives a copy of the row start, never read.  The
-// loads are tracked by vmcnt, so the barrier that ends a block's step also
-// completes the next block's staging.
-template <int NT>
+// Stages the 16x16 tiles K in [k0, k1) of row block J of the factors
+// (logical rows 16J..16J+15 of M) into buf, tile-major (tile K at buf +
+// 256 K, row-major inside), and the 16x16 diagonal block of the saved 32x32
+// inverse `inv` into dbuf, with direct global-to-LDS loads
+// (global_load_lds_dwordx4: lane l's 16 bytes land at dst + 16 l).  A
+// 1 KiB chunk is half a tile: lane l reads columns 2 (l % 8) and the next
+// of tile row 8 h + l / 8.  Only the tiles a step's update reads are
+// staged: K > J in the first solve, K < J in the second.  Chunk c goes to
+// wavefront c % 4, so a wavefront always loads the same half h = wave % 2
+// of the tiles, from one row of M.  The two chunks of the diagonal inverse
+// follow.  The loads are tracked by vmcnt, so the barrier that ends a
+// block's step also completes the next block's staging.
 __device__ __forceinline__ void solve_stage(const double* M, int ldm, const int* lab, const double* inv, int J,
-                                            double* buf, double* dbuf) {
-    constexpr int LDBK = NT * 16 + 16;
-    constexpr int NCH = 2 * (NT + 1);
+                                            int k0, int k1, double* buf, double* dbuf) {
     const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
-    for (int c = wave; c < NCH + 2; c += NWAVE) {
-        if (c < NCH) {
-            const int o = c * 128 + lane * 2;
-            const int r = o / LDBK, col = o % LDBK;
-            const double* row = M + (size_t)lab[16 * J + r] * ldm;
-            __builtin_amdgcn_global_load_lds(col < NT * 16 ? row + col : row, (lds_void*)(buf + c * 128), 16, 0, 0);
-        } else {
-            const int o = (c - NCH) * 128 + lane * 2;
-            const int h = (J & 1) * 16;
-            const double* src = inv + (size_t)(J >> 1) * RB * RB + (h + o / 16) * RB + h + o % 16;
-            __builtin_amdgcn_global_load_lds(src, (lds_void*)(dbuf + (c - NCH) * 128), 16, 0, 0);
-        }
+    const int nch = k1 > k0 ? 2 * (k1 - k0) : 0;
+    const int h = wave & 1;
+    const double* row = M + (size_t)lab[16 * J + 8 * h + (lane >> 3)] * ldm + 2 * (lane & 7);
+    for (int c = wave; c < nch; c += NWAVE) {
+        const int K = k0 + (c >> 1);
+        __builtin_amdgcn_global_load_lds(row + 16 * K, (lds_void*)(buf + K * 256 + h * 128), 16, 0, 0);
+    }
+    const int d = (wave - nch) & (NWAVE - 1);   // chunk nch + d of this wavefront
+    if (d < 2) {
+        const int o = d * 128 + lane * 2;
+        const int hb = (J & 1) * 16;
+        const double* src = inv + (size_t)(J >> 1) * RB * RB + (hb + o / 16) * RB + hb + o % 16;
+        __builtin_amdgcn_global_load_lds(src, (lds_void*)(dbuf + d * 128), 16, 0, 0);
     }
 }
 
@@ -723,9 +725,9 @@ __device__ __forceinline__ void solve_stage(const double* M, int ldm, const int*
 //     T_J = (U1_JJ^-1)^T T_J,  then T_K -= U1[J,K]^T T_J for K > J;
 //   x L1 = y  <=>  L1^T x^T = y^T, J descending:
 //     T_J = (L1_JJ^-1)^T T_J,  then T_K -= L1[J,K]^T T_J for K < J.
-// Row block J of the factors (16 x N) and the 16x16 diagonal inverse are
-// staged in LDS, double buffered: the next block streams in while the
-// current one is used, one barrier per block.  Step q of the 2 nt steps
+// The tiles of row block J that the update reads and the 16x16 diagonal
+// inverse are staged in LDS, double buffered: the next block streams in
+// while the current one is used, one barrier per block.  Step q of the 2 nt steps
 // reads buffer q & 1; q = J in the first solve and 2 nt - 1 - J in the
 // second, so the buffer is (J + 1) & 1 there, known at compile time.
 //
@@ -737,8 +739,7 @@ __device__ __forceinline__ void solve_stage(const double* M, int ldm, const int*
 template <typename T, int NT>
 __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2)))
 k_refresh_solve(RefreshArgs a, int groups) {
-    constexpr int LDBK = NT * 16 + 16;   // LDS row stride: row groups 1 and 2 land 32 banks apart
-    __shared__ __align__(16) double sb0[16 * LDBK], sb1[16 * LDBK];
+    __shared__ __align__(16) double sb0[NT * 256], sb1[NT * 256];   // tile-major row blocks
     __shared__ __align__(16) double sd0[256], sd1[256];
     extern __shared__ __align__(16) unsigned char lds[];
     const int b = (int)blockIdx.x;
@@ -805,10 +806,11 @@ k_refresh_solve(RefreshArgs a, int groups) {
             tile[J][g] = (site >= 0 && c < N) ? -Ucm[(size_t)c * ns + site] : 0.0;
         }
     }
-    // A operands of block column K of the staged row block
+    // A operands of tile K of the staged row block (64 consecutive doubles
+    // per k-slice: no bank conflicts)
     auto lds_a = [&](const double* bb, int K, double (&v)[4]) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) v[s] = bb[(4 * s + kr) * LDBK + 16 * K + kc];
+        for (int s = 0; s < 4; ++s) v[s] = bb[K * 256 + (4 * s + kr) * 16 + kc];
     };
     // T_J = y = (-inv)^T T_J from the staged diagonal inverse
     auto diag = [&](const double* db, int J) -> f64x4 {
@@ -840,7 +842,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
         }
     };
 
-    solve_stage<NT>(M, ldm, lab, Usave, 0, sb0, sd0);
+    solve_stage(M, ldm, lab, Usave, 0, 1, NT, sb0, sd0);
     __syncthreads();
     // y U1 = b, block rows ascending (step J reads buffer J & 1).  Blocks
     // K >= nt are zero padding, so the updates run to NT without guards.
@@ -849,8 +851,8 @@ k_refresh_solve(RefreshArgs a, int groups) {
         if (J < nt) {
             double* nb = (J & 1) ? sb0 : sb1;
             double* nd = (J & 1) ? sd0 : sd1;
-            if (J + 1 < nt) solve_stage<NT>(M, ldm, lab, Usave, J + 1, nb, nd);
-            else solve_stage<NT>(M, ldm, lab, Lsave, nt - 1, nb, nd);
+            if (J + 1 < nt) solve_stage(M, ldm, lab, Usave, J + 1, J + 2, NT, nb, nd);
+            else solve_stage(M, ldm, lab, Lsave, nt - 1, 0, nt - 1, nb, nd);
             const double* bb = (J & 1) ? sb1 : sb0;
             const double* db = (J & 1) ? sd1 : sd0;
             if (active) {
@@ -868,7 +870,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
 #pragma unroll
     for (int J = NT - 1; J >= 0; --J) {
         if (J < nt) {
-            if (J > 0) solve_stage<NT>(M, ldm, lab, Lsave, J - 1, (J & 1) ? sb1 : sb0, (J & 1) ? sd1 : sd0);
+            if (J > 0) solve_stage(M, ldm, lab, Lsave, J - 1, 0, J - 1, (J & 1) ? sb1 : sb0, (J & 1) ? sd1 : sd0);
             const double* bb = (J & 1) ? sb0 : sb1;
             const double* db = (J & 1) ? sd0 : sd1;
             if (active) {
@@ -940,7 +942,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
 }
 
 // LDS of k_refresh_solve: static row-block buffers plus the dynamic part.
-static size_t solve_static_lds(int nt) { return (size_t)(2 * 16 * (nt * 16 + 16) + 2 * 256) * sizeof(double); }
+static size_t solve_static_lds(int nt) { return (size_t)(2 * nt * 256 + 2 * 256) * sizeof(double); }
 static size_t solve_lds_bytes(int ns, int nt) { return (size_t)(nt * 16 + NWAVE + ns) * sizeof(int); }
 
 size_t refresh_ws_doubles_per_job(int ns, int nmax) { return refresh_ws_job(ns, nmax); }
