@@ -762,7 +762,9 @@ k_refresh_solve(RefreshArgs a, int groups) {
     const int* __restrict__ jint = reinterpret_cast<const int*>(Usave + (size_t)refresh_npanel(nmax) * RB * RB);
     if (jint[0] != 1) return;                     // factorisation rejected (status already set)
     const int* __restrict__ kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
-    const double* __restrict__ Ucm = spin ? a.Udn : a.Uup;     // ns x N column-major
+    // ns x N row-major: a lane's four labels 16J + kr + 4g of one site lie in
+    // one 128-byte line, shared by the kr lanes of that site
+    const double* __restrict__ UT = spin ? a.UTdn : a.UTup;
     const int t = (int)threadIdx.x, wave = t / 64, lane = t % 64;
 
     int* lab = reinterpret_cast<int*>(lds);       // [NT*16] logical row -> physical row of M
@@ -803,7 +805,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c = 16 * J + kr + 4 * g;
-            tile[J][g] = (site >= 0 && c < N) ? -Ucm[(size_t)c * ns + site] : 0.0;
+            tile[J][g] = (site >= 0 && c < N) ? -UT[(size_t)site * N + c] : 0.0;
         }
     }
     // A operands of tile K of the staged row block (64 consecutive doubles
