@@ -746,8 +746,7 @@ k_refresh_solve(RefreshArgs a, int groups) {
     const int job = ((b >> 3) / groups) * 8 + (b & 7);
     const int grp = (b >> 3) % groups;
     if (job >= a.njobs) return;
-    if (a.count_dev && job >= 2 * (*a.count_dev - a.walker_off)) return;
-    const int w = a.jobs[job >> 1], spin = job & 1;
+    const int spin = job & 1;
     const int ns = a.ns, N = spin ? a.ndn : a.nup, nmax = a.nmax;
     const int nE = ns - N;
     // rows stored: complex orbitals keep the first ns_out sites only
@@ -760,7 +759,13 @@ k_refresh_solve(RefreshArgs a, int groups) {
     const double* __restrict__ Lsave = M + (size_t)refresh_nrows(ns) * ldm;
     const double* __restrict__ Usave = Lsave + (size_t)refresh_npanel(nmax) * RB * RB;
     const int* __restrict__ jint = reinterpret_cast<const int*>(Usave + (size_t)refresh_npanel(nmax) * RB * RB);
-    if (jint[0] != 1) return;                     // factorisation rejected (status already set)
+    // the job's count, walker and factorisation flag in one memory round
+    // trip (every address is in bounds for job < njobs)
+    const int cnt = a.count_dev ? *a.count_dev : 0;
+    const int w = a.jobs[job >> 1];
+    const int fact = jint[0];
+    if (a.count_dev && job >= 2 * (cnt - a.walker_off)) return;
+    if (fact != 1) return;                        // factorisation rejected (status already set)
     const int* __restrict__ kap = (spin ? a.kdn : a.kup) + (size_t)w * ns;
     // ns x N row-major: a lane's four labels 16J + kr + 4g of one site lie in
     // one 128-byte line, shared by the kr lanes of that site
@@ -771,12 +776,27 @@ k_refresh_solve(RefreshArgs a, int groups) {
     int* wsum = lab + NT * 16;                    // [NWAVE]
     int* esite = wsum + NWAVE;                    // [ns] empty sites in order
 
-    // rows N..16 nt - 1 are the zeroed padding rows of M
-    for (int c = t; c < NT * 16; c += RT) lab[c] = c < N ? jint[1 + c] : c;
+    // rows N..16 nt - 1 are the zeroed padding rows of M; the labels and
+    // the first sites' kappa are loaded together, and each pass of the
+    // ballot loads the next pass's kappa before its barriers
+    int labv[(NT * 16 + RT - 1) / RT];
+#pragma unroll
+    for (int i = 0; i < (NT * 16 + RT - 1) / RT; ++i) {
+        const int c = t + i * RT;
+        labv[i] = c < N ? jint[1 + c] : c;
+    }
+    int lnext = t < ns ? kap[t] : -1;
+#pragma unroll
+    for (int i = 0; i < (NT * 16 + RT - 1) / RT; ++i) {
+        const int c = t + i * RT;
+        if (c < NT * 16) lab[c] = labv[i];
+    }
     int base = 0;
     for (int s0 = 0; s0 < ns; s0 += RT) {
         const int s = s0 + t;
-        const bool empty = s < ns && kap[s] == 0;
+        const int l = lnext;
+        lnext = s + RT < ns ? kap[s + RT] : -1;
+        const bool empty = s < ns && l == 0;
         const unsigned long long m = __ballot(empty);
         const int before = __popcll(m & ((1ull << lane) - 1ull));
         if (lane == 0) wsum[wave] = __popcll(m);
@@ -791,6 +811,14 @@ k_refresh_solve(RefreshArgs a, int groups) {
         __syncthreads();
     }
 
+    // the sites whose slot map this workgroup rewrites (write mode): their
+    // first pass of kappa is loaded now, with the tiles
+    int s_lo = 0, s_hi = 0, kfirst = 1;
+    if (a.write_w) {
+        s_lo = grp == 0 ? 0 : esite[64 * grp];
+        s_hi = 64 * (grp + 1) < nEo ? esite[64 * (grp + 1)] : nso;
+        kfirst = s_lo + t < s_hi ? kap[s_lo + t] : 1;
+    }
     const int e = 64 * grp + 16 * wave + (lane & 15);
     const int site = e < nEo ? esite[e] : -1;
     const bool active = 64 * grp + 16 * wave < nEo;   // wavefront holds at least one site
@@ -932,9 +960,8 @@ k_refresh_solve(RefreshArgs a, int groups) {
         }
     }
     if (a.write_w) {
-        const int s_lo = grp == 0 ? 0 : esite[64 * grp];
-        const int s_hi = 64 * (grp + 1) < nEo ? esite[64 * (grp + 1)] : nso;
-        for (int s = s_lo + t; s < s_hi; s += RT)
+        if (s_lo + t < s_hi && kfirst != 0) sl[s_lo + t] = -1;
+        for (int s = s_lo + t + RT; s < s_hi; s += RT)
             if (kap[s] != 0) sl[s] = -1;
         if (t < 64 && 64 * grp + t < nEo) sl[esite[64 * grp + t]] = 64 * grp + t;
     } else {
