@@ -30,6 +30,7 @@
 #include "kvmc_internal.hpp"
 
 #include <math.h>
+#include <type_traits>
 
 namespace kvmc {
 
@@ -459,47 +460,88 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
             }
         }
         __syncthreads();
-        // trailing update: M[r][c] -= L21[r][:] U12[:][c]
+        // trailing update: M[r][c] -= L21[r][:] U12[:][c].  A wavefront
+        // takes its row tiles TG at a time, so each load of the U12 operands
+        // (read back from M through L2; staged in LDS they took 102 KB at
+        // N = 432, one workgroup per CU) feeds TG independent MFMA chains,
+        // and the next column tile's operands are loaded while the current
+        // one is updated.  The C operands come from U in the first panel
+        // and from M afterwards (a loop for each, so that only one set of
+        // row offsets is live).
+        constexpr int TG = 1;
         const int r0 = c0;
         const int ntr = (nrl - r0) / 16;
-        for (int ti = wave; ti < ntr; ti += NWAVE) {
-            const int pa = phys[r0 + 16 * ti + (lane & 15)];
-            double af[RB / 4];
+        int uofs[RB / 4];   // U12 operand rows: offsets in M
 #pragma unroll
-            for (int s = 0; s < RB / 4; ++s) af[s] = -M[(size_t)pa * ldm + j0 + 4 * s + (lane >> 4)];
-            double* crow[4];
-            int prow[4];
+        for (int s = 0; s < RB / 4; ++s) uofs[s] = phys[j0 + 4 * s + (lane >> 4)] * ldm + c0 + (lane & 15);
+        auto tiles = [&](auto first_panel) {
+            constexpr bool FIRST = decltype(first_panel)::value;
+            for (int tb = wave; tb < ntr; tb += TG * NWAVE) {
+                double af[TG][RB / 4];
+                int mofs[TG][4];   // C rows: offsets in M
+                int cofs[TG][4];   // first panel: the rows' offsets in U, -1 for padding rows
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                prow[g] = phys[r0 + 16 * ti + (lane >> 4) + 4 * g];
-                crow[g] = M + (size_t)prow[g] * ldm + c0 + (lane & 15);
-            }
-            // U12 operand rows, read back from M through L2 (staged in LDS
-            // they took 102 KB at N = 432, one workgroup per CU)
-            const double* urow[RB / 4];
+                for (int k = 0; k < TG; ++k) {
+                    // a tile past ntr repeats the last one and is never stored
+                    const int ti = min(tb + k * NWAVE, ntr - 1);
+                    const int pa = phys[r0 + 16 * ti + (lane & 15)];
 #pragma unroll
-            for (int s = 0; s < RB / 4; ++s) urow[s] = M + (size_t)phys[j0 + 4 * s + (lane >> 4)] * ldm + c0;
-            // C operands: from U in the first panel, from M afterwards
-            auto cval = [&](int g, int tj) -> double {
-                return j0 == 0 ? u_at(prow[g], c0 + 16 * tj + (lane & 15)) : crow[g][16 * tj];
-            };
-            f64x4 acc, nxt;
+                    for (int s = 0; s < RB / 4; ++s) af[k][s] = -M[(size_t)pa * ldm + j0 + 4 * s + (lane >> 4)];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) acc[g] = cval(g, 0);
-            for (int tj = 0; tj < ntc; ++tj) {
-                const int cc = 16 * tj + (lane & 15);
-                if (tj + 1 < ntc) {
+                    for (int g = 0; g < 4; ++g) {
+                        const int pr = phys[r0 + 16 * ti + (lane >> 4) + 4 * g];
+                        mofs[k][g] = pr * ldm + c0 + (lane & 15);
+                        if constexpr (FIRST) cofs[k][g] = pr < (FULL ? N + nE : N) ? rowsite[pr] * N : -1;
+                    }
+                }
+                auto cval = [&](int k, int g, int tj) -> double {
+                    if constexpr (FIRST) {
+                        const int col = c0 + 16 * tj + (lane & 15);
+                        return (cofs[k][g] >= 0 && col < N) ? UT[(size_t)cofs[k][g] + col] : 0.0;
+                    } else {
+                        return M[(size_t)mofs[k][g] + 16 * tj];
+                    }
+                };
+                f64x4 acc[TG], nxt[TG];
+                double ub[RB / 4], un[RB / 4];
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) nxt[g] = cval(g, tj + 1);
+                for (int k = 0; k < TG; ++k) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) acc[k][g] = cval(k, g, 0);
                 }
 #pragma unroll
-                for (int s = 0; s < RB / 4; ++s)
-                    acc = mfma16(af[s], urow[s][cc], acc);
+                for (int s = 0; s < RB / 4; ++s) ub[s] = M[uofs[s]];
+                for (int tj = 0; tj < ntc; ++tj) {
+                    if (tj + 1 < ntc) {
 #pragma unroll
-                for (int g = 0; g < 4; ++g) crow[g][16 * tj] = acc[g];
-                acc = nxt;
+                        for (int k = 0; k < TG; ++k) {
+#pragma unroll
+                            for (int g = 0; g < 4; ++g) nxt[k][g] = cval(k, g, tj + 1);
+                        }
+#pragma unroll
+                        for (int s = 0; s < RB / 4; ++s) un[s] = M[uofs[s] + 16 * (tj + 1)];
+                    }
+#pragma unroll
+                    for (int s = 0; s < RB / 4; ++s) {
+#pragma unroll
+                        for (int k = 0; k < TG; ++k) acc[k] = mfma16(af[k][s], ub[s], acc[k]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < TG; ++k) {
+                        if (tb + k * NWAVE < ntr) {
+#pragma unroll
+                            for (int g = 0; g < 4; ++g) M[(size_t)mofs[k][g] + 16 * tj] = acc[k][g];
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < TG; ++k) acc[k] = nxt[k];
+#pragma unroll
+                    for (int s = 0; s < RB / 4; ++s) ub[s] = un[s];
+                }
             }
-        }
+        };
+        if (j0 == 0) tiles(std::true_type{});
+        else tiles(std::false_type{});
         __syncthreads();
     }
     double logdet = 0.0, pmin = INFINITY, pmax = 0.0;
