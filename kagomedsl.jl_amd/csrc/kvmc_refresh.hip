@@ -461,13 +461,14 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         }
         __syncthreads();
         // trailing update: M[r][c] -= L21[r][:] U12[:][c].  A wavefront
-        // takes its row tiles TG at a time, so each load of the U12 operands
+        // updates TG row tiles at a time: each load of the U12 operands
         // (read back from M through L2; staged in LDS they took 102 KB at
-        // N = 432, one workgroup per CU) feeds TG independent MFMA chains,
-        // and the next column tile's operands are loaded while the current
-        // one is updated.  The C operands come from U in the first panel
-        // and from M afterwards (a loop for each, so that only one set of
-        // row offsets is live).
+        // N = 432, one workgroup per CU) feeds TG MFMA chains, and the next
+        // column tile's operands are loaded while the current one is
+        // updated.  (TG = 2 needs about 200 VGPRs and spills at three waves
+        // per SIMD.)  The C operands come from U in the first panel and from
+        // M afterwards, a loop for each, so that only one set of row
+        // offsets is live.
         constexpr int TG = 1;
         const int r0 = c0;
         const int ntr = (nrl - r0) / 16;
