@@ -69,16 +69,19 @@ def test_refresh_many_walkers(oracle_mod, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_refresh_above_256_electrons(oracle_mod, prec):
+@pytest.mark.parametrize("n_up", [264, 240])
+def test_refresh_above_256_electrons(oracle_mod, prec, n_up):
     """N_up = 264 at L=12 (432 sites, N_down = 168): beyond 256 electrons of
     one spin the refresh runs as a single kernel over [tildeU; U_empty].
-    Walker 0 starts from the pivoted-QR configuration, walker 1 from it with
-    20 seeded up/down exchanges, each kept only while both tildeU stay
-    well conditioned (this filling is not a closed shell, so U, and with it
-    the conditioning of a given configuration, depends on the eigensolver)."""
+    N_up = 240 (N_down = 192) is the widest two-kernel case: 15 column
+    blocks, the solve kernel's 16-tile instance.  Walker 0 starts from the
+    pivoted-QR configuration, walker 1 from it with 20 seeded up/down
+    exchanges, each kept only while both tildeU stay well conditioned (these
+    fillings are not closed shells, so U, and with it the conditioning of a
+    given configuration, depends on the eigensolver)."""
     from kagome_vmc import DoubleKagome, Hamiltonian, init_conf_qr
     lat = DoubleKagome.build(1.0, 12, 12, (True, True), (False, True))
-    ham = Hamiltonian.build(264, 168, lat)
+    ham = Hamiltonian.build(n_up, 432 - n_up, lat)
     ku0, kd0 = init_conf_qr(ham.U_up, ham.U_down)
     ku1, kd1 = ku0.copy(), kd0.copy()
     rng = np.random.default_rng(5)
