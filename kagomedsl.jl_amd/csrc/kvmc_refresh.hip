@@ -440,23 +440,69 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
             Lsave[(size_t)(j0 / RB) * RB * RB + idx] = Linv[(idx / RB) * LB + idx % RB];
         const int c0 = j0 + RB;
         if (c0 >= nc16) continue;
-        // U12 = L11^-1 A12 on MFMA: one 16-column block per wavefront
+        // U12 = L11^-1 A12 on MFMA: one 16-column block per wavefront at a
+        // time.  The single kernel at one or two panel rows per thread (two
+        // waves per SIMD) loads the next block's A12 operands while the
+        // current one is formed (bofs: the rows' offsets in U in the first
+        // panel, -1 for padding rows, in M afterwards); elsewhere that
+        // spills, and the operands are loaded per block.
         const int ntc = (nc16 - c0) / 16;
-        for (int tj = wave; tj < ntc; tj += NWAVE) {
-            const int cc = 16 * tj + (lane & 15);
-            f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+        if constexpr (FULL && RPT <= 2) {
+            int bofs[RB / 4];
 #pragma unroll
             for (int s = 0; s < RB / 4; ++s) {
-                const int k = 4 * s + (lane >> 4);
-                const double b = j0 == 0 ? u_at(phys[k], c0 + cc) : M[(size_t)phys[j0 + k] * ldm + c0 + cc];
-                u0 = mfma16(Linv[(lane & 15) * LB + k], b, u0);
-                u1 = mfma16(Linv[(16 + (lane & 15)) * LB + k], b, u1);
+                const int pr = phys[j0 + 4 * s + (lane >> 4)];
+                bofs[s] = j0 == 0 ? (pr < N + nE ? rowsite[pr] * N : -1) : pr * ldm + c0 + (lane & 15);
             }
+            auto bval = [&](int s, int tj) -> double {
+                const int col = c0 + 16 * tj + (lane & 15);
+                if (j0 == 0) return (bofs[s] >= 0 && col < N) ? UT[(size_t)bofs[s] + col] : 0.0;
+                return M[(size_t)bofs[s] + 16 * tj];
+            };
+            double bcur[RB / 4], bnext[RB / 4];
+            if (wave < ntc) {
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int r0 = (lane >> 4) + 4 * g, r1 = r0 + 16;
-                M[(size_t)phys[j0 + r0] * ldm + c0 + cc] = u0[g];
-                M[(size_t)phys[j0 + r1] * ldm + c0 + cc] = u1[g];
+                for (int s = 0; s < RB / 4; ++s) bcur[s] = bval(s, wave);
+            }
+            for (int tj = wave; tj < ntc; tj += NWAVE) {
+                const int cc = 16 * tj + (lane & 15);
+                if (tj + NWAVE < ntc) {
+#pragma unroll
+                    for (int s = 0; s < RB / 4; ++s) bnext[s] = bval(s, tj + NWAVE);
+                }
+                f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) {
+                    const int k = 4 * s + (lane >> 4);
+                    u0 = mfma16(Linv[(lane & 15) * LB + k], bcur[s], u0);
+                    u1 = mfma16(Linv[(16 + (lane & 15)) * LB + k], bcur[s], u1);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int r0 = (lane >> 4) + 4 * g, r1 = r0 + 16;
+                    M[(size_t)phys[j0 + r0] * ldm + c0 + cc] = u0[g];
+                    M[(size_t)phys[j0 + r1] * ldm + c0 + cc] = u1[g];
+                }
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) bcur[s] = bnext[s];
+            }
+        } else {
+            for (int tj = wave; tj < ntc; tj += NWAVE) {
+                const int cc = 16 * tj + (lane & 15);
+                f64x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) {
+                    const int k = 4 * s + (lane >> 4);
+                    const double b = j0 == 0 ? u_at(phys[k], c0 + cc) : M[(size_t)phys[j0 + k] * ldm + c0 + cc];
+                    u0 = mfma16(Linv[(lane & 15) * LB + k], b, u0);
+                    u1 = mfma16(Linv[(16 + (lane & 15)) * LB + k], b, u1);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int r0 = (lane >> 4) + 4 * g, r1 = r0 + 16;
+                    M[(size_t)phys[j0 + r0] * ldm + c0 + cc] = u0[g];
+                    M[(size_t)phys[j0 + r1] * ldm + c0 + cc] = u1[g];
+                }
             }
         }
         __syncthreads();
