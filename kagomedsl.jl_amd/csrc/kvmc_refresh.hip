@@ -107,6 +107,33 @@ __device__ __attribute__((noinline)) void unit_lower_inverse(const double* L, do
     }
 }
 
+// 16 x 16 diagonal blocks at (off, off) of the same inverses, for the
+// two-kernel path, one block per wavefront: lanes 0..15 of wavefront `wv`
+// each hold one column.  Rows >= jb of the block are unit rows.
+__device__ __attribute__((noinline)) void unit_lower_inverse16(const double* L, double* Linv, int off, int jb, int wv) {
+    if ((int)threadIdx.x / 64 != wv) return;
+    const int j = (int)threadIdx.x & 15;
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        double v = (i == j) ? 1.0 : 0.0;
+        if (i > j && off + i < jb) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < i; k += 2) {
+                s0 += L[(off + i) * LB + off + k] * x[k];
+                if (k + 1 < i) s1 += L[(off + i) * LB + off + k + 1] * x[k + 1];
+            }
+            v = -(s0 + s1);
+        }
+        x[i] = v;
+    }
+    if ((threadIdx.x & 63) < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Linv[(off + i) * LB + off + j] = x[i];
+    }
+}
+
 __device__ __forceinline__ unsigned long long readlane64(unsigned long long k, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, l);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), l);
@@ -144,6 +171,28 @@ __device__ __attribute__((noinline)) void upper_inverse(const double* D, double*
     if (lane < RB) {
 #pragma unroll
         for (int i = 0; i < RB; ++i) Uinv[i * ld + j] = x[i];
+    }
+}
+
+__device__ __attribute__((noinline)) void upper_inverse16(const double* D, double* Uinv, int ld, int off, int wv) {
+    if ((int)threadIdx.x / 64 != wv) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int j = lane & 15;
+    const double dinv = 1.0 / D[(off + j) * LB + off + j];
+    double x[16];
+#pragma unroll
+    for (int i = 15; i >= 0; --i) {
+        double s0 = (i == j) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 16; k += 2) {
+            s0 -= D[(off + i) * LB + off + k] * x[k];
+            if (k + 1 < 16) s1 -= D[(off + i) * LB + off + k + 1] * x[k + 1];
+        }
+        x[i] = (s0 + s1) * readlane_f64(dinv, i);
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Uinv[(off + i) * ld + off + j] = x[i];
     }
 }
 
@@ -407,9 +456,33 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         // U11^-1: kept for the solve kernel (computed by wavefront 0 while
         // wavefront 1 forms L11^-1), or applied here to the empty-site rows,
         // whose panel slice solves x U11 = a  ->  x = a U11^-1
-        if (!FULL) {
-            upper_inverse(Dblk, Usave + (size_t)(j0 / RB) * RB * RB, RB, 0);
-            unit_lower_inverse(Dblk, Linv, jb, 1);
+        if constexpr (!FULL) {
+            // the 16x16 diagonal blocks of U11^-1 (all the solve kernel reads)
+            // and of L11^-1, one per wavefront; then the lower-left block of
+            // L11^-1, -Lb^-1 Lc La^-1, on MFMA (Lc: rows 16.., columns ..15 of
+            // L11); the upper-right block is zero
+            double* Us = Usave + (size_t)(j0 / RB) * RB * RB;
+            upper_inverse16(Dblk, Us, RB, 0, 0);
+            upper_inverse16(Dblk, Us, RB, 16, 1);
+            unit_lower_inverse16(Dblk, Linv, 0, jb, 2);
+            unit_lower_inverse16(Dblk, Linv, 16, jb, 3);
+            __syncthreads();
+            if (wave == 0) {
+                f64x4 tq = {0, 0, 0, 0}, xq = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int k = 4 * s + (lane >> 4);
+                    tq = mfma16(Dblk[(16 + (lane & 15)) * LB + k], Linv[k * LB + (lane & 15)], tq);
+                }
+                // the product's accumulator layout is the B-operand layout
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    xq = mfma16(Linv[(16 + (lane & 15)) * LB + 16 + 4 * s + (lane >> 4)], tq[s], xq);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) Linv[(16 + (lane >> 4) + 4 * g) * LB + (lane & 15)] = -xq[g];
+            } else if (wave == 1) {
+                for (int idx = lane; idx < 256; idx += 64) Linv[(idx / 16) * LB + 16 + idx % 16] = 0.0;
+            }
             __syncthreads();
         } else {
             upper_inverse(Dblk, Linv, LB, 0);
