@@ -584,11 +584,12 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         // (read back from M through L2; staged in LDS they took 102 KB at
         // N = 432, one workgroup per CU) feeds TG MFMA chains, and the next
         // column tile's operands are loaded while the current one is
-        // updated.  (TG = 2 needs about 200 VGPRs and spills at three waves
-        // per SIMD.)  The C operands come from U in the first panel and from
-        // M afterwards, a loop for each, so that only one set of row
-        // offsets is live.
-        constexpr int TG = 1;
+        // updated.  TG = 2 needs about 200 VGPRs: it fits the single kernel at
+        // two panel rows per thread (two waves per SIMD, under the panel's
+        // register peak) and spills elsewhere.  The C operands come from U in
+        // the first panel and from M afterwards, a loop for each, so that
+        // only one set of row offsets is live.
+        constexpr int TG = (FULL && RPT == 2) ? 2 : 1;
         const int r0 = c0;
         const int ntr = (nrl - r0) / 16;
         int uofs[RB / 4];   // U12 operand rows: offsets in M
