@@ -712,46 +712,68 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         for (int idx = t; idx < RB * RB; idx += RT)
             Linv[(idx / RB) * LB + idx % RB] = Lsave[(size_t)pq * RB * RB + idx];
         __syncthreads();
-        // X_J = B_J L1_JJ^-1, one 16-row tile per wavefront (both column tiles)
+        // X_J = B_J L1_JJ^-1, one 16-row tile per wavefront (both column
+        // tiles), the next tile's B operands loaded while this one is formed
         const int ntr = m16 / 16;
-        for (int ti = wave; ti < ntr; ti += NWAVE) {
-            const int ra = N + 16 * ti + (lane & 15);
-            f64x4 x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};
+        {
+            double bcur[RB / 4], bnext[RB / 4];
+            auto bload = [&](int ti, double (&v)[RB / 4]) {
+                const int ra = N + 16 * ti + (lane & 15);
 #pragma unroll
-            for (int s = 0; s < RB / 4; ++s) {
-                const int k = 4 * s + (lane >> 4);
-                const double av = M[(size_t)ra * ldm + j0 + k];
-                x0 = mfma16(av, Linv[k * LB + (lane & 15)], x0);
-                x1 = mfma16(av, Linv[k * LB + 16 + (lane & 15)], x1);
-            }
+                for (int s = 0; s < RB / 4; ++s) v[s] = M[(size_t)ra * ldm + j0 + 4 * s + (lane >> 4)];
+            };
+            if (wave < ntr) bload(wave, bcur);
+            for (int ti = wave; ti < ntr; ti += NWAVE) {
+                if (ti + NWAVE < ntr) bload(ti + NWAVE, bnext);
+                f64x4 x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int rr = 16 * ti + (lane >> 4) + 4 * g;
-                M[(size_t)(N + rr) * ldm + j0 + (lane & 15)] = x0[g];
-                M[(size_t)(N + rr) * ldm + j0 + 16 + (lane & 15)] = x1[g];
+                for (int s = 0; s < RB / 4; ++s) {
+                    const int k = 4 * s + (lane >> 4);
+                    x0 = mfma16(bcur[s], Linv[k * LB + (lane & 15)], x0);
+                    x1 = mfma16(bcur[s], Linv[k * LB + 16 + (lane & 15)], x1);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int rr = 16 * ti + (lane >> 4) + 4 * g;
+                    M[(size_t)(N + rr) * ldm + j0 + (lane & 15)] = x0[g];
+                    M[(size_t)(N + rr) * ldm + j0 + 16 + (lane & 15)] = x1[g];
+                }
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s) bcur[s] = bnext[s];
             }
         }
         __syncthreads();
-        // M[N.., 0:j0] -= X_J L1[J, 0:j0]
+        // M[N.., 0:j0] -= X_J L1[J, 0:j0]: a wavefront takes two row tiles at
+        // a time, which share each load of the L1 operand rows, and loads the
+        // next column tile's operands while the current one is updated
+        constexpr int TS = 2;
         const int ntc = j0 / 16;
-        for (int ti = wave; ti < ntr; ti += NWAVE) {
-            double af[RB / 4];
+        int pb[RB / 4];
 #pragma unroll
-            for (int s = 0; s < RB / 4; ++s)
-                af[s] = -M[(size_t)(N + 16 * ti + (lane & 15)) * ldm + j0 + 4 * s + (lane >> 4)];
-            int pb[RB / 4];
+        for (int s = 0; s < RB / 4; ++s) {
+            const int k = 4 * s + (lane >> 4);
+            pb[s] = k < jb ? phys[j0 + k] : -1;
+        }
+        for (int tb = wave; tb < ntr; tb += TS * NWAVE) {
+            double af[TS][RB / 4];
+            int cofs[TS];   // offset in M of the tile row this lane updates first
 #pragma unroll
-            for (int s = 0; s < RB / 4; ++s) {
-                const int k = 4 * s + (lane >> 4);
-                pb[s] = k < jb ? phys[j0 + k] : -1;
+            for (int q = 0; q < TS; ++q) {
+                // a tile past ntr repeats the last one and is never stored
+                const int ti = min(tb + q * NWAVE, ntr - 1);
+#pragma unroll
+                for (int s = 0; s < RB / 4; ++s)
+                    af[q][s] = -M[(size_t)(N + 16 * ti + (lane & 15)) * ldm + j0 + 4 * s + (lane >> 4)];
+                cofs[q] = (N + 16 * ti + (lane >> 4)) * ldm + (lane & 15);
             }
-            const int rc = N + 16 * ti + (lane >> 4);
-            double* crow = M + (size_t)rc * ldm + (lane & 15);
-            f64x4 acc, nxt;
+            f64x4 acc[TS], nxt[TS];
             double bf[RB / 4], bn[RB / 4];
             if (ntc > 0) {
 #pragma unroll
-                for (int g = 0; g < 4; ++g) acc[g] = crow[(size_t)4 * g * ldm];
+                for (int q = 0; q < TS; ++q) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) acc[q][g] = M[(size_t)cofs[q] + (size_t)4 * g * ldm];
+                }
 #pragma unroll
                 for (int s = 0; s < RB / 4; ++s) bf[s] = pb[s] >= 0 ? M[(size_t)pb[s] * ldm + (lane & 15)] : 0.0;
             }
@@ -759,16 +781,28 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
                 if (tj + 1 < ntc) {
                     const int cn = 16 * (tj + 1);
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) nxt[g] = crow[(size_t)4 * g * ldm + cn];
+                    for (int q = 0; q < TS; ++q) {
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) nxt[q][g] = M[(size_t)cofs[q] + (size_t)4 * g * ldm + cn];
+                    }
 #pragma unroll
                     for (int s = 0; s < RB / 4; ++s)
                         bn[s] = pb[s] >= 0 ? M[(size_t)pb[s] * ldm + cn + (lane & 15)] : 0.0;
                 }
 #pragma unroll
-                for (int s = 0; s < RB / 4; ++s) acc = mfma16(af[s], bf[s], acc);
+                for (int s = 0; s < RB / 4; ++s) {
 #pragma unroll
-                for (int g = 0; g < 4; ++g) crow[(size_t)4 * g * ldm + 16 * tj] = acc[g];
-                acc = nxt;
+                    for (int q = 0; q < TS; ++q) acc[q] = mfma16(af[q][s], bf[s], acc[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < TS; ++q) {
+                    if (tb + q * NWAVE < ntr) {
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) M[(size_t)cofs[q] + (size_t)4 * g * ldm + 16 * tj] = acc[q][g];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < TS; ++q) acc[q] = nxt[q];
 #pragma unroll
                 for (int s = 0; s < RB / 4; ++s) bf[s] = bn[s];
             }
@@ -799,18 +833,43 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
     const T* gA = reinterpret_cast<const T*>(a.A) + ((size_t)w * 2 + spin) * nsa * dp * cw;
     const T* gB = reinterpret_cast<const T*>(a.Bm) + ((size_t)w * 2 + spin) * a.nmax_out * dp * cw;
     double dmax = 0.0, vmax = N > 0 ? 1.0 : 0.0;   // occupied rows: unit rows, exact
+    // Site s < nso is empty exactly when its row is one of X's (rowof[s] >=
+    // N).  A wavefront holds rows wave and wave + 4 of a block in registers
+    // (N <= RPT * RT columns): the next block's are loaded while the current
+    // block is written.
+    constexpr int RPW = (SB + NWAVE - 1) / NWAVE, CPL = RPT * RT / 64;
+    double pre[RPW][CPL];
+    auto load_rows = [&](int sb) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int i = wave + r * NWAVE, s = sb + i;
+            const bool ok = i < SB && s < nso && rowof[s] >= N;
+            const double* __restrict__ src = M + (size_t)(ok ? rowof[s] : 0) * ldm;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const int col = lane + 64 * c;
+                pre[r][c] = (ok && col < N) ? src[col] : 0.0;
+            }
+        }
+    };
+    load_rows(0);
     for (int sb = 0; sb < nso; sb += SB) {
-        for (int i = wave; i < SB; i += NWAVE) {
-            const int s = sb + i;
-            if (s < nso && kap[s] == 0) {
-                const double* __restrict__ src = M + (size_t)rowof[s] * ldm;
-                for (int c = lane; c < N; c += 64) xb[i * xs + c] = src[c];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int i = wave + r * NWAVE, s = sb + i;
+            if (i < SB && s < nso && rowof[s] >= N) {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const int col = lane + 64 * c;
+                    if (col < N) xb[i * xs + col] = pre[r][c];
+                }
             }
         }
         __syncthreads();
+        if (sb + SB < nso) load_rows(sb + SB);
         for (int idx = t; idx < N * SB; idx += RT) {
             const int l = idx / SB, i = idx % SB, s = sb + i;
-            if (s >= nso || kap[s] != 0) continue;
+            if (s >= nso || rowof[s] < N) continue;
             const int part = l >= NO ? 1 : 0, lo = l - part * NO;
             const double v = part ? -xb[i * xs + pinv[l]] : xb[i * xs + pinv[l]];
             const int row = a.write_w ? rowof[s] - N : max(sl[s], 0);
@@ -835,7 +894,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(FULL ? 
         __syncthreads();
     }
     if (a.write_w) {
-        for (int s = t; s < nso; s += RT) sl[s] = kap[s] == 0 ? rowof[s] - N : -1;
+        for (int s = t; s < nso; s += RT) sl[s] = rowof[s] >= N ? rowof[s] - N : -1;
     } else {
         atomic_max_nonneg2(&a.drift[(size_t)w * 2], dmax);
         atomic_max_nonneg2(&a.drift[(size_t)w * 2 + 1], vmax);
