@@ -618,6 +618,28 @@ __device__ __forceinline__ typename AccT<T>::type w_elem(const T* W0, const T* g
     return v;
 }
 
+// The pending terms' part of w_elem, added to v (W0[l][K] as loaded).
+template <typename T>
+__device__ __forceinline__ typename AccT<T>::type w_terms(typename AccT<T>::type v, const T* gA, const T* gB,
+                                                          int dp, int K, int l, int c) {
+    const T* ra = gA + (size_t)K * dp;
+    const T* rb = gB + (size_t)l * dp;
+    if constexpr (!is_cplx<T>::value) {
+        constexpr int NV = 16 / (int)sizeof(T);
+        using VT = typename VecT<T, NV>::type;
+        for (int q0 = 0; q0 < c; q0 += NV) {
+            const VT x = *reinterpret_cast<const VT*>(ra + q0);
+            const VT y = *reinterpret_cast<const VT*>(rb + q0);
+#pragma unroll
+            for (int u = 0; u < NV; ++u)
+                if (q0 + u < c) v += (double)(x[u] * y[u]);
+        }
+    } else {
+        for (int q = 0; q < c; ++q) v += to_acc(ra[q] * rb[q]);
+    }
+    return v;
+}
+
 // off -= 1/2 x y for the exchange term (real part), offi likewise (imaginary).
 __device__ __forceinline__ void exch_acc(double& off, double&, double x, double y) { off += -0.5 * x * y; }
 __device__ __forceinline__ void exch_acc(double& off, double& offi, cf64 x, cf64 y) {
@@ -752,19 +774,43 @@ __global__ void __launch_bounds__(256) k_eloc_wg(ElocArgs a) {
     }
     __syncthreads();
     double diag = 0.0, off = 0.0, offi = 0.0, nap = 0.0;
-    for (int b = t0; b < a.nb; b += bd) {
-        const int2 bb = a.bonds[b];
-        const int i = bb.x, j = bb.y;
-        const int iu = sku[i], ju = sku[j], id = skd[i], jd = skd[j];
-        if (ju != 0 && id != 0)   // S+_i S-_j; the target sites are empty for the moving spin
-            exch_acc(off, offi, w_elem(Wup, sAu, sBu, ldu, dp, max((int)ssu[i], 0), ju - 1, c),
-                     w_elem(Wdn, sAd, sBd, ldd, dp, max((int)ssd[j], 0), id - 1, c));
-        if (iu != 0 && jd != 0)   // S-_i S+_j
-            exch_acc(off, offi, w_elem(Wup, sAu, sBu, ldu, dp, max((int)ssu[j], 0), iu - 1, c),
-                     w_elem(Wdn, sAd, sBd, ldd, dp, max((int)ssd[i], 0), jd - 1, c));
-        nap += ((iu != 0 && jd != 0) || (ju != 0 && id != 0)) ? 1.0 : 0.0;
-        const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
-        diag += szi * szj;
+    // A thread's bonds (BPT per pass) issue all their W gathers before any
+    // is used: a bond with no move gathers W[0][0] and discards it.  Every
+    // site holds one electron, so at most one of the two exchanges
+    // (S+_i S-_j: i down, j up; S-_i S+_j: i up, j down) applies, and its
+    // target sites are empty for the moving spin.
+    constexpr int BPT = 4;
+    for (int b0 = t0; b0 < a.nb; b0 += BPT * bd) {
+        int Ku[BPT], Lu[BPT], Kd[BPT], Ld[BPT];
+        bool mv[BPT];
+        T w0u[BPT], w0d[BPT];
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int b = b0 + u * bd;
+            const bool in = b < a.nb;
+            const int2 bb = in ? a.bonds[b] : make_int2(0, 0);
+            const int i = bb.x, j = bb.y;
+            const int iu = sku[i], ju = sku[j], id = skd[i], jd = skd[j];
+            const bool t1 = ju != 0 && id != 0, t2 = iu != 0 && jd != 0;
+            mv[u] = in && (t1 || t2);
+            Lu[u] = mv[u] ? (t1 ? ju : iu) - 1 : 0;
+            Ku[u] = mv[u] ? max((int)(t1 ? ssu[i] : ssu[j]), 0) : 0;
+            Ld[u] = mv[u] ? (t1 ? id : jd) - 1 : 0;
+            Kd[u] = mv[u] ? max((int)(t1 ? ssd[j] : ssd[i]), 0) : 0;
+            w0u[u] = Wup[(size_t)Lu[u] * ldu + Ku[u]];
+            w0d[u] = Wdn[(size_t)Ld[u] * ldd + Kd[u]];
+            if (in) {
+                nap += (t1 || t2) ? 1.0 : 0.0;
+                const double szi = iu != 0 ? 0.5 : -0.5, szj = ju != 0 ? 0.5 : -0.5;
+                diag += szi * szj;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            if (mv[u])
+                exch_acc(off, offi, w_terms(to_acc(w0u[u]), sAu, sBu, dp, Ku[u], Lu[u], c),
+                         w_terms(to_acc(w0d[u]), sAd, sBd, dp, Kd[u], Ld[u], c));
+        }
     }
     const double v = block_sum_double(diag + off, red);
     const double napt = block_sum_double(nap, red + 8);
