@@ -755,6 +755,9 @@ __global__ void __launch_bounds__(256) k_eloc_wg(ElocArgs a) {
     T* sBd = sBu + (size_t)nmax * dp;
     double* red = reinterpret_cast<double*>(sBu + (size_t)2 * nmax * dp);
     const int t0 = (int)threadIdx.x, bd = (int)blockDim.x;
+    // copies unrolled so that several loads are in flight before the first
+    // LDS store waits on them
+#pragma unroll 2
     for (int t = t0; t < ns; t += bd) {
         sku[t] = (int16_t)a.kup[(size_t)w * ns + t];
         skd[t] = (int16_t)a.kdn[(size_t)w * ns + t];
@@ -769,7 +772,9 @@ __global__ void __launch_bounds__(256) k_eloc_wg(ElocArgs a) {
         const f32x4* gB = reinterpret_cast<const f32x4*>(reinterpret_cast<const T*>(a.Bm) + (size_t)w * 2 * nmax * dp);
         f32x4* dA = reinterpret_cast<f32x4*>(sAu);
         f32x4* dB = reinterpret_cast<f32x4*>(sBu);
+#pragma unroll 4
         for (size_t t = t0; t < na; t += bd) dA[t] = gA[t];
+#pragma unroll 4
         for (size_t t = t0; t < nbv; t += bd) dB[t] = gB[t];
     }
     __syncthreads();
