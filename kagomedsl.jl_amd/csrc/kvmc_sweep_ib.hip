@@ -154,6 +154,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <
     int acc_since = a.acc_since[w];
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
 
+    // The copies into LDS are unrolled so that several loads are in flight
+    // before the first store waits on them.
+#pragma unroll 2
     for (int t = tid; t < ns; t += bd) {
         s_kup[t] = (int16_t)a.kup[(size_t)w * ns + t];
         s_kdn[t] = (int16_t)a.kdn[(size_t)w * ns + t];
@@ -161,12 +164,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <
         s_sld[t] = (int16_t)slot_g[ns + t];
     }
     if (c > 0) {
-        // the At columns ([spin][row][dp] in HBM) and the term rows and labels
+        // the At columns ([spin][row][dp] in HBM, dp a multiple of 4: 16-byte
+        // loads) and the term rows and labels
+#pragma unroll 2
         for (int t = tid; t < 2 * nsa; t += bd) {
             const int sp = t >= nsa ? 1 : 0;
             float* d = s_At + sp * delay * nsa + (t - sp * nsa);
-            const float* g = gA + (size_t)t * dp;
-            for (int q = 0; q < c; ++q) d[q * nsa] = g[q];
+            const f32x4* g = reinterpret_cast<const f32x4*>(gA + (size_t)t * dp);
+            f32x4 v[(DMAX + 3) / 4];
+#pragma unroll
+            for (int q4 = 0; q4 < (DMAX + 3) / 4; ++q4)
+                if (4 * q4 < c) v[q4] = g[q4];
+#pragma unroll
+            for (int q4 = 0; q4 < (DMAX + 3) / 4; ++q4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (4 * q4 + u < c) d[(4 * q4 + u) * nsa] = v[q4][u];
+            }
         }
         for (int t = tid; t < 2 * c; t += bd) {
             const int sp = t / c, q = t % c;
@@ -174,19 +188,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <
             s_L[sp * delay + q] = pt[(sp * kMaxDelay + q) * 2 + 1];
         }
     }
+#pragma unroll 4
     for (int b = tid; b < nb; b += bd) {
         const int2 bb = a.bonds[b];
         s_bond[b] = (uint32_t)bb.x | ((uint32_t)bb.y << 16);
     }
+#pragma unroll 2
     for (int t = tid; t <= ns; t += bd) s_noff[t] = (uint16_t)a.nbr_off[t];
-    for (int t = tid; t < a.nbr_off[ns]; t += bd) s_nbr[t] = (uint16_t)a.nbr[t];
+    const int nnbr = a.nbr_off[ns];
+#pragma unroll 4
+    for (int t = tid; t < nnbr; t += bd) s_nbr[t] = (uint16_t)a.nbr[t];
     __syncthreads();
     int zmu;
     {
         double part = 0.0;
         for (int b = tid; b < nb; b += bd) {
-            const int2 bb = a.bonds[b];
-            part += (s_kup[bb.x] != 0 && s_kdn[bb.y] != 0) || (s_kup[bb.y] != 0 && s_kdn[bb.x] != 0);
+            const uint32_t bp = s_bond[b];
+            const int i = (int)(bp & 0xffffu), j = (int)(bp >> 16);
+            part += (s_kup[i] != 0 && s_kdn[j] != 0) || (s_kup[j] != 0 && s_kdn[i] != 0);
         }
         zmu = (int)block_sum_double(part, s_red);
     }
@@ -377,19 +396,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DMAX <
     // fails keeps a W consistent with its configuration
     const bool listed = pause || (a.policy == POLICY_AT_END && acc_since >= a.refresh_k);
     if (c > 0) {
+        // written as 16-byte vectors (slots c.. of the last one get zeros;
+        // readers use the first c); all of a label's gathers are issued
+        // before the first is stored
+        constexpr int NV4 = (DMAX + 3) / 4;
         for (int t = tid; t < 2 * nsa; t += bd) {
             const int sp = t >= nsa ? 1 : 0;
             const float* d = s_At + sp * delay * nsa + (t - sp * nsa);
-            float* g = gA + (size_t)t * dp;
-            for (int q = 0; q < c; ++q) g[q] = d[q * nsa];
+            f32x4* g = reinterpret_cast<f32x4*>(gA + (size_t)t * dp);
+#pragma unroll
+            for (int q4 = 0; q4 < NV4; ++q4) {
+                if (4 * q4 < c) {
+                    f32x4 v;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v[u] = 4 * q4 + u < c ? d[(4 * q4 + u) * nsa] : 0.f;
+                    g[q4] = v;
+                }
+            }
         }
         // the rows g_q = W0[K_q, :] - e_{l_q}, one label per thread
         for (int t = tid; t < 2 * nmax; t += bd) {
             const int sp = t >= nmax ? 1 : 0, l = t - sp * nmax;
             if (l >= (sp ? ndn : nup)) continue;
             const float* col = (sp ? Wdn : Wup) + (size_t)l * (sp ? ldd : ldu);
-            float* g = gB + (size_t)t * dp;
-            for (int q = 0; q < c; ++q) g[q] = col[s_K[sp * delay + q]] - (l == s_L[sp * delay + q] ? 1.f : 0.f);
+            float v[4 * NV4];
+#pragma unroll
+            for (int q = 0; q < 4 * NV4; ++q)
+                v[q] = q < c ? col[s_K[sp * delay + q]] - (l == s_L[sp * delay + q] ? 1.f : 0.f) : 0.f;
+            f32x4* g = reinterpret_cast<f32x4*>(gB + (size_t)t * dp);
+#pragma unroll
+            for (int q4 = 0; q4 < NV4; ++q4)
+                if (4 * q4 < c) g[q4] = f32x4{v[4 * q4], v[4 * q4 + 1], v[4 * q4 + 2], v[4 * q4 + 3]};
         }
         for (int t = tid; t < 2 * c; t += bd) {
             const int sp = t / c, q = t % c;
