@@ -48,7 +48,7 @@ FP64_PEAK_TFLOPS = 78.6
 # walker refreshed by the kernel pair k_refresh_blocked + k_refresh_solve, from
 # rocprofv3 --pmc passes over this script's default FP32 run (summaries in
 # profiles/).
-REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 6_153_990}
+REFRESH_TRAFFIC_BYTES_PER_WALKER = {12: 6_157_947}
 # the same for the FP32 sweep kernel (k_sweep_ib), per walker-sweep (delay 12,
 # L=12, refresh every 216 accepts)
 SWEEP_TRAFFIC_BYTES_PER_WALKER_SWEEP = {12: 10_394}
