@@ -592,13 +592,11 @@ hipError_t launch_flush_t(const SweepArgs& a, int nw, hipStream_t s) {
 // reference keeps the real part; complex W also yields the imaginary part
 // (needed by the energy gradient of a complex wave function).
 // ---------------------------------------------------------------------------
-// The c pending terms of row K and label l are contiguous (pend_ld apart
-// per row).
+// The pending terms' part of W_cur[K][l], added to v: the c terms of row K
+// and of label l are contiguous (pend_ld apart per row).
 template <typename T>
-__device__ __forceinline__ typename AccT<T>::type w_elem(const T* W0, const T* gA, const T* gB, int ld, int dp,
-                                                         int K, int l, int c) {
-    using A = typename AccT<T>::type;
-    A v = to_acc(W0[(size_t)l * ld + K]);
+__device__ __forceinline__ typename AccT<T>::type w_terms(typename AccT<T>::type v, const T* gA, const T* gB,
+                                                          int dp, int K, int l, int c) {
     const T* ra = gA + (size_t)K * dp;
     const T* rb = gB + (size_t)l * dp;
     if constexpr (!is_cplx<T>::value) {
@@ -618,26 +616,11 @@ __device__ __forceinline__ typename AccT<T>::type w_elem(const T* W0, const T* g
     return v;
 }
 
-// The pending terms' part of w_elem, added to v (W0[l][K] as loaded).
+// W_cur[K][l]: the flushed W0 (label-major, row stride ld) plus the terms.
 template <typename T>
-__device__ __forceinline__ typename AccT<T>::type w_terms(typename AccT<T>::type v, const T* gA, const T* gB,
-                                                          int dp, int K, int l, int c) {
-    const T* ra = gA + (size_t)K * dp;
-    const T* rb = gB + (size_t)l * dp;
-    if constexpr (!is_cplx<T>::value) {
-        constexpr int NV = 16 / (int)sizeof(T);
-        using VT = typename VecT<T, NV>::type;
-        for (int q0 = 0; q0 < c; q0 += NV) {
-            const VT x = *reinterpret_cast<const VT*>(ra + q0);
-            const VT y = *reinterpret_cast<const VT*>(rb + q0);
-#pragma unroll
-            for (int u = 0; u < NV; ++u)
-                if (q0 + u < c) v += (double)(x[u] * y[u]);
-        }
-    } else {
-        for (int q = 0; q < c; ++q) v += to_acc(ra[q] * rb[q]);
-    }
-    return v;
+__device__ __forceinline__ typename AccT<T>::type w_elem(const T* W0, const T* gA, const T* gB, int ld, int dp,
+                                                         int K, int l, int c) {
+    return w_terms<T>(to_acc(W0[(size_t)l * ld + K]), gA, gB, dp, K, l, c);
 }
 
 // off -= 1/2 x y for the exchange term (real part), offi likewise (imaginary).
