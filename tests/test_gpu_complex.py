@@ -202,6 +202,35 @@ def test_complex_refresh_single_kernel_path(oracle_mod, L, fillings):
         np.testing.assert_array_equal(kf[0][w], ch.kup)
 
 
+def test_complex_drift_single_kernel_path(oracle_mod):
+    """Drift mode of the single-kernel refresh (complex L=10, real form
+    2N = 300): with drift tracking on, every policy refresh inside
+    kvmc_sweep first compares the carried FP32 W with fresh rows.  The count
+    equals the refresh count, the drift is positive and small, and with
+    tracking off the chains and energies are unchanged (the drift pass
+    writes nothing)."""
+    from kagome_vmc import _native as nat
+    ham = complex_system(10)
+    nw = 4
+    ku, kd = conditioned(oracle_mod, ham, nw, 7)
+    res = {}
+    for track in (True, False):
+        with sampler(ham, nw, precision=nat.PREC_FP32, delay=12, refresh_policy=nat.REFRESH_EVERY_K,
+                     refresh_k=16, seed=5) as ds:
+            ds.set_configs(ku, kd)
+            ds.track_drift(track)
+            cnt, _ = ds.sweep(300)
+            res[track] = (cnt, ds.drift_stats(), ds.get_configs(), ds.measure()[0])
+    cnt, (dmax, dmean, n), k_on, e_on = res[True]
+    assert n == cnt["refreshes"] > 0
+    assert 0.0 < dmean <= dmax < 1e-3
+    cnt_off, stats_off, k_off, e_off = res[False]
+    assert stats_off[2] == 0
+    assert np.array_equal(k_on[0], k_off[0]) and np.array_equal(k_on[1], k_off[1])
+    assert np.array_equal(e_on, e_off)
+    print(f"complex L=10 drift before refresh: max {dmax:.2e}, mean {dmean:.2e} over {n}")
+
+
 def test_mixed_random_flux_groups_follow_oracle(oracle_mod):
     """The mixed-size batch with a seeded flux phase per lattice (L = 4, 6,
     8, 10, 12; complex W in FP32 with delay 12, W in LDS for L <= 6 and
