@@ -462,18 +462,29 @@ def run_mixed(args):
 
 def run_grad(args):
     """Gradient throughput: walkers from |psi|^2 (unbiased kernel, FP64 W) on
-    the ansatz family at theta0, one step = N sweeps + one local-energy and
-    log-derivative pass; reports measure_grad evaluations/s and the sampled
-    gradient with its jackknife-free standard error over walkers."""
+    the ansatz family at theta = (phi, ratio, m, t2), drawn uniformly from
+    [0, pi] x [0.8, 1.2] x [0, 0.2] x [-0.1, 0.1] with the seed (complex
+    orbitals unless phi = 0), or theta0 = (0, 1, 0, 0) with --theta theta0.
+    One step = N sweeps + one local-energy and log-derivative pass; reports
+    measure_grad evaluations/s, the sampled <O_k>, <E_L conj(O_k)>, the SR
+    matrix S and the gradient with its standard error over walkers."""
     from kagome_vmc import _native as nat
-    from kagome_vmc.gradient import THETA0, GradientSampler, ansatz_family
+    from kagome_vmc.gradient import THETA0, GradientSampler, ansatz_family, averages
+    from kagome_vmc.hamiltonian import shell_gap
     from kagome_vmc.lattice import DoubleKagome
     nat.load()
     L = args.L
     pbc, apbc = BC[L]
     fam = ansatz_family(DoubleKagome.build(1.0, L, L, pbc, apbc))
     n = 3 * L * L // 2
-    gs = GradientSampler(fam, THETA0, n, n, args.walkers, seed=args.seed, precision=nat.PREC_FP64)
+    if args.theta == "seed":
+        rng = np.random.default_rng(args.seed)
+        theta = np.array([rng.uniform(0.0, np.pi), rng.uniform(0.8, 1.2), rng.uniform(0.0, 0.2),
+                          rng.uniform(-0.1, 0.1)])
+    else:
+        theta = THETA0.copy()
+    gap = shell_gap(np.linalg.eigvalsh(fam.hamiltonian(theta)), n)
+    gs = GradientSampler(fam, theta, n, n, args.walkers, seed=args.seed, precision=nat.PREC_FP64)
     s = gs.sampler
     s.init_random(args.seed)
     for _ in range(args.warmup):     # warm-up steps do the timed steps' work (first launches included)
@@ -491,21 +502,31 @@ def run_grad(args):
         E.append(e)
         O.append(o)
     wall = time.perf_counter() - t0
+    is_cplx = bool(gs.sampler.complex)
     gs.close()
     E, O = np.stack(E), np.stack(O)
     # complex log-derivatives (the flux direction): dE = 2 Re(<O* E_L> - E <O*>)
     w_e = E.real.mean(0)
     Oc = np.conj(O)
     w_g = 2.0 * ((E[..., None] * Oc).mean(0) - E.mean(0)[:, None] * Oc.mean(0)).real
+    avg = averages(E.ravel(), O.reshape(-1, O.shape[-1]))
+    cpair = lambda z: [[float(np.real(v)), float(np.imag(v))] for v in np.ravel(z)]
     line = {
         "metric": f"gradient evaluations/s (E_L and 4 log-derivatives per walker), L={L}",
         "value": args.walkers * args.steps / t_grad, "unit": "walker-evaluations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"pi-flux Kagome L={L}, {args.walkers} walkers from |psi|^2, "
-                               f"theta0 = (phi, ratio, m, t2) = (0, 1, 0, 0); step = {n} sweeps + measure_grad"},
+        "config": {"workload": f"Kagome L={L} ({3 * L * L} sites), {args.walkers} walkers from |psi|^2 at "
+                               f"theta = (phi, ratio, m, t2) = ({', '.join(f'{v:.6g}' for v in theta)}) "
+                               f"({'drawn with the seed' if args.theta == 'seed' else 'theta0'}); "
+                               f"step = {n} sweeps + measure_grad",
+                   "theta": theta.tolist(), "complex_orbitals": is_cplx,
+                   "fermi_shell_gap": gap},
         "energy_per_site": float(w_e.mean() / (3 * L * L)),
+        "O_mean_re_im": cpair(avg["O"]),
+        "E_conjO_mean_re_im": cpair(avg["EO"]),
+        "S": avg["S"].tolist(),
         "gradient_per_site": (w_g.mean(0) / (3 * L * L)).tolist(),
         "gradient_stderr_per_site": (w_g.std(0, ddof=1) / np.sqrt(args.walkers) / (3 * L * L)).tolist(),
         "walker_sweeps_per_s": args.walkers * n * args.steps / wall,
@@ -542,6 +563,8 @@ def main():
                     help="untimed steps after the timed region, each checking the carried W")
     ap.add_argument("--graph", action="store_true",
                     help="replay captured sweep sequences (no per-kernel times)")
+    ap.add_argument("--theta", default="seed", choices=("seed", "theta0"),
+                    help="gradient form: parameters drawn with the seed, or theta0 = (0, 1, 0, 0)")
     ap.add_argument("--grad", action="store_true",
                     help="only run the gradient workload (unbiased kernel, FP64, log-derivatives)")
     ap.add_argument("--mixed", action="store_true",
