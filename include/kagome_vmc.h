@@ -109,7 +109,10 @@ typedef struct kvmc_counters {
 
 /* MC(params) + the W/kappa allocation of the low-level constructor
  * (src/MonteCarlo.jl:165-185, 209-234).  Copies U and the bonds to the
- * device and allocates per-walker state for desc->nwalkers walkers. */
+ * device and allocates per-walker state for desc->nwalkers walkers.
+ * KVMC_ERR_INVALID (with the reason) for sizes the refresh does not take:
+ * more than 768 electrons per spin (384 with complex orbitals), or an LDS
+ * footprint beyond 160 KiB. */
 int kvmc_create(const kvmc_desc* desc, kvmc_handle** out);
 
 /* Frees all device memory of the handle. */

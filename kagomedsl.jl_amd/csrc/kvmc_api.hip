@@ -457,7 +457,8 @@ int kvmc_create(const kvmc_desc* d, kvmc_handle** out) {
     const int cf = d->orbitals == KVMC_ORB_COMPLEX ? 2 : 1;   // the refresh factors a 2N x 2N real form
     if (d->ns > 4096) return fail(KVMC_ERR_INVALID, "ns > 4096 is not supported");
     if (!refresh_supported(cf * d->ns, cf * d->n_up, cf * d->n_down))
-        return fail(KVMC_ERR_INVALID, "lattice too large for the refresh kernel (LDS / rows per thread)");
+        return fail(KVMC_ERR_INVALID, "lattice too large for the refresh kernel (LDS, or more than 768 "
+                                      "electrons per spin; 384 with complex orbitals)");
     for (int b = 0; b < d->nb; ++b) {
         int i = d->bonds[2 * b], j = d->bonds[2 * b + 1];
         if (i < 1 || j < 1 || i > d->ns || j > d->ns || i > j)

@@ -1273,12 +1273,17 @@ static hipError_t launch_refresh_t(const RefreshArgs& a, size_t lds, hipStream_t
         case 1: return launch_lu<T, 1, true>(a, lds, s);
         case 2: return launch_lu<T, 2, true>(a, lds, s);
         case 3: return launch_lu<T, 3, true>(a, lds, s);
-        default: return launch_lu<T, 4, true>(a, lds, s);
+        default: return hipErrorInvalidValue;   // rejected by refresh_supported
     }
 }
 
+// At most three panel rows per thread (N <= 768, i.e. 2N <= 768 for the real
+// form of complex orbitals).  A four-row instance (about 700 spilled VGPRs)
+// reported a zero pivot for a complex L = 18 start with cond(tildeU) < 1e6
+// that the oracle factors; the cause was not found, so such sizes are
+// rejected when the handle is created.
 bool refresh_supported(int ns, int nup, int ndn) {
-    return refresh_lds_bytes(ns, nup, ndn) <= 160 * 1024 && refresh_rows_per_thread(nup > ndn ? nup : ndn) <= 4;
+    return refresh_lds_bytes(ns, nup, ndn) <= 160 * 1024 && refresh_rows_per_thread(nup > ndn ? nup : ndn) <= 3;
 }
 
 hipError_t launch_refresh(int prec, const RefreshArgs& a, hipStream_t s) {

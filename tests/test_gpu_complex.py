@@ -231,6 +231,38 @@ def test_complex_drift_single_kernel_path(oracle_mod):
     print(f"complex L=10 drift before refresh: max {dmax:.2e}, mean {dmean:.2e} over {n}")
 
 
+def test_complex_refresh_largest_supported_size(oracle_mod):
+    """Complex L=16 (768 sites, 384 electrons per spin): the real form has
+    2N = 768 columns, the most the single-kernel refresh takes (three panel
+    rows per thread for every row of tildeU).  W and log|det| of one walker
+    against the oracle (a start with cond(tildeU) below 1e6, so the bar is
+    1e-8), then a short FP64 tape chain."""
+    from kagome_vmc import DoubleKagome, Hamiltonian
+    from kagome_vmc.hamiltonian import flux_link_tables
+    lat = DoubleKagome.build(1.0, 16, 16, (True, True), (False, False))
+    tin, tint = flux_link_tables(lat, PHI)
+    ham = Hamiltonian.build(384, 384, lat, tin, tint, B=B)
+    nw, nsw = 1, 60
+    ku, kd = conditioned(oracle_mod, ham, nw, 5, max_cond=1e6)
+    tu, tb = tape(nw, nsw, len(ham.nn), 9)
+    with sampler(ham, nw) as ds:
+        ds.set_configs(ku, kd)
+        logdet = ds.logdet()
+        Wu, lu = oracle_mod.refresh_w(ham.U_up, ku[0], with_logdet=True)
+        Wd, ld = oracle_mod.refresh_w(ham.U_down, kd[0], with_logdet=True)
+        eu, ed = _max_rel(ds.get_w(0, 0), Wu), _max_rel(ds.get_w(0, 1), Wd)
+        print(f"complex L=16 refresh: W max rel {max(eu, ed):.2e}")
+        assert eu < 1e-8 and ed < 1e-8
+        assert abs(logdet[0] - (lu + ld)) < 1e-9 * max(1.0, abs(lu + ld))
+        cnt, out = ds.sweep(nsw, tu, tb, record=True)
+        kf = ds.get_configs()
+    ch = oracle_mod.Chain.start(ham.nn, ham.U_up, ham.U_down, ku[0], kd[0])
+    oc, _, _ = oracle_mod.run_chain(ch, nsw, tape_u=tu[0], tape_bond=tb[0])
+    np.testing.assert_array_equal(out[0], oc)
+    np.testing.assert_array_equal(kf[0][0], ch.kup)
+    np.testing.assert_array_equal(kf[1][0], ch.kdn)
+
+
 def test_mixed_random_flux_groups_follow_oracle(oracle_mod):
     """The mixed-size batch with a seeded flux phase per lattice (L = 4, 6,
     8, 10, 12; complex W in FP32 with delay 12, W in LDS for L <= 6 and

@@ -63,3 +63,27 @@ def test_invalid_descriptor_is_rejected_before_any_gpu_work(lib):
     rc = lib.kvmc_create(ctypes.byref(d), ctypes.byref(h))
     assert rc == _native.KVMC_ERR_INVALID
     assert b"n_up + n_down" in lib.kvmc_last_error()
+
+
+def test_refresh_size_limit_is_checked_before_any_gpu_work(lib):
+    """More than 768 rows of tildeU in the refresh (here complex orbitals at
+    972 sites, 486 electrons per spin: a 972 x 972 real form) is rejected
+    when the handle is created, with the reason."""
+    import numpy as np
+    from kagome_vmc import _native
+    ns, n = 972, 486
+    bonds = np.array([[1, 2]], dtype=np.int32)
+    u = np.zeros(2 * ns * n)
+    d = _native.KvmcDesc()
+    d.ns, d.n_up, d.n_down, d.nb = ns, n, n, 1
+    d.bonds = bonds.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    d.u_up = u.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    d.u_down = u.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    d.nwalkers = 1
+    d.precision, d.kernel, d.refresh_policy = _native.PREC_FP64, _native.KERNEL_REFERENCE, _native.REFRESH_REFERENCE
+    d.delay = 1
+    d.orbitals = _native.ORB_COMPLEX
+    h = ctypes.c_void_p()
+    rc = lib.kvmc_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _native.KVMC_ERR_INVALID
+    assert b"too large for the refresh kernel" in lib.kvmc_last_error()
