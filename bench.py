@@ -386,19 +386,33 @@ MIXED_GROUPS = ((4, 4096), (6, 2048), (8, 2048), (10, 1024), (12, 1024))
 
 def run_mixed(args):
     """All groups of MIXED_GROUPS (L, walkers) at once, one handle and host
-    thread per group, FP32 W with delay and FP64 refresh every `k` accepts.
+    thread per group, FP32 W with delayed updates and FP64 refresh every `k`
+    accepts; each group's workgroup size and delay come from the autotuner.
     Each lattice takes a flux phase phi drawn uniformly from [0, pi] with the
     seed (complex orbitals, complex W).  One step = n_occupied sweeps + one
     local-energy pass in every group.  Also times each group alone, for the
     overlap gain and for each group's kernel times and HBM roofline."""
     from kagome_vmc import _native as nat
+    from kagome_vmc.autotune import autotune
     from kagome_vmc.batch import MixedBatch, flux_cluster
     nat.load()
     rng = np.random.default_rng(args.seed)
     phis = [float(rng.uniform(0.0, np.pi)) for _ in MIXED_GROUPS]
-    groups = [(flux_cluster(L, phi), nw) for (L, nw), phi in zip(MIXED_GROUPS, phis)]
-    kw = dict(precision=nat.PREC_FP32, refresh_policy=nat.REFRESH_EVERY_K, refresh_k=args.k,
-              delay=args.delay, seed=args.seed)
+    hams = [flux_cluster(L, phi) for (L, _), phi in zip(MIXED_GROUPS, phis)]
+    # launch shape per group (workgroup size and delay) from the autotuner,
+    # each candidate timed on the group's own workload; --no-autotune keeps
+    # --delay and the library's default workgroup size for every group
+    shapes = []
+    for ham, (_, nw) in zip(hams, MIXED_GROUPS):
+        if args.no_autotune:
+            shapes.append({"threads": 0, "delay": args.delay})
+        else:
+            r = autotune(ham, nw, nat.PREC_FP32, args.k, seed=args.seed, steps=2, cache_path=None)
+            shapes.append({"threads": r["threads"], "delay": r["delay"]})
+        log(f"[mixed] L={int(round((ham.ns / 3) ** 0.5))}: threads {shapes[-1]['threads']}, "
+            f"delay {shapes[-1]['delay']}")
+    groups = [(ham, nw, shp) for ham, (_, nw), shp in zip(hams, MIXED_GROUPS, shapes)]
+    kw = dict(precision=nat.PREC_FP32, refresh_policy=nat.REFRESH_EVERY_K, refresh_k=args.k, seed=args.seed)
 
     def timed(batch, profile=False):
         batch.init_random(args.seed)
@@ -424,8 +438,8 @@ def run_mixed(args):
         wall_threads, res_threads, _ = timed(mb)
     per = []
     alone_total = 0.0
-    for (ham, nw), (cnt, _), phi in zip(groups, res, phis):
-        with MixedBatch([(ham, nw)], **kw) as one:
+    for (ham, nw, shp), (cnt, _), phi in zip(groups, res, phis):
+        with MixedBatch([(ham, nw, shp)], **kw) as one:
             t1, r1, prof = timed(one, profile=True)
         alone_total += t1
         c1 = r1[0][0]
@@ -433,6 +447,7 @@ def run_mixed(args):
         b = sweep_bytes(c1, s, ham.ns, ham.N_up, ham.N_down, nw)
         sweep_gbs = b / (prof[0]["sweep_ms"] * 1e-3) / 1e9 if prof[0]["sweep_ms"] > 0 else None
         per.append({"L": int(round((ham.ns / 3) ** 0.5)), "walkers": nw, "phi": phi,
+                    "threads": shp["threads"], "delay": shp["delay"],
                     "walker_sweeps": cnt["sweeps"],
                     "alone_walker_sweeps_per_s": c1["sweeps"] / t1, "refreshes": cnt["refreshes"],
                     # both W of a walker within 64 KiB stay in LDS (empty-site rows only)
@@ -450,8 +465,10 @@ def run_mixed(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "mixed: " + ", ".join(f"L={L} x {nw}" for L, nw in MIXED_GROUPS)
-                               + f"; seeded flux phase phi in [0, pi] per lattice (complex W); FP32 W delay "
-                               f"{args.delay}, FP64 refresh every {args.k} accepts",
+                               + f"; seeded flux phase phi in [0, pi] per lattice (complex W); FP32 W with "
+                               + ("the autotuner's workgroup size and delay per group" if not args.no_autotune
+                                  else f"delay {args.delay}")
+                               + f", FP64 refresh every {args.k} accepts",
                    "groups": per},
         "sequential_walker_sweeps_per_s": total / alone_total,
         "launch": "one graph launch per step for all groups" if not args.threads_mixed else "one host thread per group",
@@ -565,6 +582,8 @@ def main():
                     help="replay captured sweep sequences (no per-kernel times)")
     ap.add_argument("--theta", default="seed", choices=("seed", "theta0"),
                     help="gradient form: parameters drawn with the seed, or theta0 = (0, 1, 0, 0)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="mixed batch: --delay and the default workgroup size for every group")
     ap.add_argument("--grad", action="store_true",
                     help="only run the gradient workload (unbiased kernel, FP64, log-derivatives)")
     ap.add_argument("--mixed", action="store_true",

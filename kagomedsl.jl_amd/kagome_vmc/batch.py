@@ -55,17 +55,23 @@ class Group:
 
 
 class MixedBatch:
-    """groups: iterable of (Hamiltonian, nwalkers).  Sampler keywords
-    (precision, refresh_policy, refresh_k, delay, kernel, seed) apply to all
-    groups; group g draws with seed + g.  graph=True sweeps all groups with
+    """groups: iterable of (Hamiltonian, nwalkers) or (Hamiltonian, nwalkers,
+    overrides).  Sampler keywords (precision, refresh_policy, refresh_k,
+    delay, threads, kernel, seed) apply to all groups; a group's overrides
+    dict (e.g. the autotuner's threads and delay) replaces them for that
+    group; group g draws with seed + g.  graph=True sweeps all groups with
     one graph launch per step."""
 
     def __init__(self, groups, graph: bool = False, **sampler_kw):
         seed = int(sampler_kw.pop("seed", 0))
         self.groups = []
-        for g, (ham, nw) in enumerate(groups):
+        for g, item in enumerate(groups):
+            ham, nw = item[0], item[1]
+            kw = dict(sampler_kw)
+            if len(item) > 2 and item[2]:
+                kw.update(item[2])
             ds = nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, int(nw),
-                                   seed=seed + g, **sampler_kw)
+                                   seed=seed + g, **kw)
             self.groups.append(Group(ham, ds))
         self._graph = nat.SamplerGroup([g.sampler for g in self.groups]) if graph else None
 

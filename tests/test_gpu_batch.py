@@ -10,15 +10,18 @@ pytestmark = pytest.mark.gpu
 
 
 def test_groups_match_their_solo_runs():
-    groups = [(pi_flux_cluster(2), 64), (pi_flux_cluster(4), 32), (pi_flux_cluster(6), 16)]
+    """The middle group carries its own launch shape (the autotuner's kind of
+    override); its solo twin is built with the same threads and delay."""
+    groups = [(pi_flux_cluster(2), 64, None), (pi_flux_cluster(4), 32, {"threads": 64, "delay": 8}),
+              (pi_flux_cluster(6), 16, None)]
     kw = dict(precision=nat.PREC_FP32, refresh_policy=nat.REFRESH_EVERY_K, refresh_k=16, delay=4,
               seed=3)
     with MixedBatch(groups, **kw) as mb:
         mb.init_random(11)
         res = mb.step(3)
         confs = [g.sampler.get_configs() for g in mb.groups]
-    for g, (ham, nw) in enumerate(groups):
-        solo_kw = dict(kw, seed=kw["seed"] + g)
+    for g, (ham, nw, shape) in enumerate(groups):
+        solo_kw = dict(kw, seed=kw["seed"] + g, **(shape or {}))
         with nat.DeviceSampler(ham.bonds_array(), ham.U_up, ham.U_down, nw, **solo_kw) as ds:
             ds.init_random(11)
             e_solo = []
