@@ -892,9 +892,112 @@ __global__ void __launch_bounds__(256) k_grad(ElocArgs a) {
     for (int p = tid; p < CW * P; p += (int)blockDim.x) a.O[(size_t)w * CW * P + p] = acc[p];
 }
 
+// The same sums taken site by site, for up to PM parameters: every site s
+// holds one electron (label l of the spin at s), so
+//   O_k = sum_s ( Q_k[s, s] + sum_r W[slot(r), l] Q_k[s, r] )
+// over the empty sites r of that spin.  A workgroup takes GW walkers with
+// four wavefronts each; wavefront (walker g, quarter q) sums the sites of
+// quarter q of walker g.  The GW wavefronts of a quarter walk its sites in
+// the same order, so each Q row is fetched from L2 once per workgroup and
+// served to the other walkers from L1.  Each W element is read once, for all
+// parameters at a time.  Labels and slot maps are staged in LDS (16-bit), so
+// a site costs one memory round trip (its W and Q loads together).
+constexpr int kGradWalkers = 4;   // walkers per workgroup of k_grad_sites
+template <typename T, int PM>
+__global__ void __launch_bounds__(1024) k_grad_sites(ElocArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    using A = typename AccT<T>::type;
+    constexpr int CW = is_cplx<T>::value ? 2 : 1;   // doubles per value
+    constexpr int GW = kGradWalkers;
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    const int g = wave % GW, q = wave / GW;          // walker slot, quarter of the sites
+    const int w = (int)blockIdx.x * GW + g;
+    const bool live = w < a.nw;                      // every wavefront reaches both barriers
+    const int ns = a.ns, nsp = round_up(ns, 4), P = a.nparams;
+    double* red = reinterpret_cast<double*>(lds);    // [GW][4][PM][CW]
+    int16_t* mk = reinterpret_cast<int16_t*>(red + GW * 4 * PM * CW) + (size_t)g * 4 * nsp;   // kup, kdn, slots
+    if (live) {
+        const int* ku = a.kup + (size_t)w * ns;
+        const int* kd = a.kdn + (size_t)w * ns;
+        const int* sg = a.slot + (size_t)w * 2 * ns;
+        for (int t = lane + 64 * q; t < ns; t += 256) {
+            mk[t] = (int16_t)ku[t];
+            mk[nsp + t] = (int16_t)kd[t];
+            mk[2 * nsp + t] = (int16_t)sg[t];
+            mk[3 * nsp + t] = (int16_t)sg[ns + t];
+        }
+    }
+    __syncthreads();
+    A part[PM];
+#pragma unroll
+    for (int p = 0; p < PM; ++p) part[p] = A(0.0);
+    if (live) {
+        const int ldu = w_ld(ns, a.nup), ldd = w_ld(ns, a.ndn);
+        const T* Wup = reinterpret_cast<const T*>(a.W) + (size_t)w * w_stride(ns, a.nup, a.ndn);
+        const T* Wdn = Wup + w_off_dn(ns, a.nup);
+        const A* Qu = reinterpret_cast<const A*>(a.Qup);
+        const A* Qd = reinterpret_cast<const A*>(a.Qdn);
+        const size_t qstride = (size_t)ns * ns;      // one parameter's Q
+        const int s0 = (int)(((long long)ns * q) / 4), s1 = (int)(((long long)ns * (q + 1)) / 4);
+        for (int s = s0; s < s1; ++s) {
+            const int lu = mk[s], lw = mk[nsp + s];
+            if (lu == 0 && lw == 0) continue;        // not a valid kappa: nothing to add
+            const bool up = lu != 0;
+            const int l = (up ? lu : lw) - 1;
+            const T* wcol = up ? Wup + (size_t)l * ldu : Wdn + (size_t)l * ldd;
+            const int16_t* sl = mk + (up ? 2 : 3) * nsp;
+            const A* qs = (up ? Qu : Qd) + (size_t)s * ns;
+            if (lane == 0) {
+#pragma unroll
+                for (int p = 0; p < PM; ++p)
+                    if (p < P) part[p] += qs[p * qstride + s];
+            }
+            for (int r = lane; r < ns; r += 64) {
+                const int e = sl[r];
+                if (e >= 0) {
+                    const A wv = to_acc(wcol[e]);
+#pragma unroll
+                    for (int p = 0; p < PM; ++p)
+                        if (p < P) part[p] += wv * qs[p * qstride + r];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PM; ++p) {
+        double pr = re_of(part[p]), pi = im_of(part[p]);
+        for (int o = 32; o > 0; o >>= 1) {
+            pr += __shfl_xor(pr, o, 64);
+            if constexpr (CW == 2) pi += __shfl_xor(pi, o, 64);
+        }
+        if (lane == 0) {
+            red[((g * 4 + q) * PM + p) * CW] = pr;
+            if constexpr (CW == 2) red[((g * 4 + q) * PM + p) * CW + 1] = pi;
+        }
+    }
+    __syncthreads();
+    if (live && q == 0 && lane < CW * PM) {
+        const int p = lane / CW, c = lane % CW;
+        if (p < P) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v += red[((g * 4 + k) * PM + p) * CW + c];
+            a.O[(size_t)w * CW * P + CW * p + c] = v;
+        }
+    }
+}
+
 template <typename T>
 hipError_t launch_grad_t(const ElocArgs& a, hipStream_t s) {
     constexpr int CW = is_cplx<T>::value ? 2 : 1;
+    if (a.nparams <= 4) {
+        constexpr int GW = kGradWalkers;
+        const size_t lds = (size_t)GW * 4 * 4 * CW * sizeof(double) +                 // partial sums
+                           (size_t)GW * 4 * ((a.ns + 3) / 4 * 4) * sizeof(int16_t);    // maps per walker
+        hipLaunchKernelGGL((k_grad_sites<T, 4>), dim3((unsigned)((a.nw + GW - 1) / GW)), dim3(64 * 4 * GW), lds, s,
+                           a);
+        return hipGetLastError();
+    }
     const size_t lds = (size_t)(2 * a.nmax + 2) * sizeof(int) + (size_t)CW * a.nparams * sizeof(double) + 16;
     hipLaunchKernelGGL((k_grad<T>), dim3(a.nw), dim3(256), lds, s, a);
     return hipGetLastError();
