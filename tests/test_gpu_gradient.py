@@ -72,6 +72,33 @@ def test_measure_grad_matches_oracle(oracle_mod, L, prec):
         assert abs(np.imag(e_ref)) > 1e-6      # complex psi: the imaginary part is live
 
 
+def test_measure_grad_beyond_four_parameters(oracle_mod):
+    """Five parameters (the four derivatives plus a fifth, 0.5 dU_0 - dU_3)
+    take the workgroup-per-walker kernel; up to four take the site-major
+    one.  Both agree with the oracle, and the fifth O equals 0.5 O_0 - O_3
+    (O is linear in dU)."""
+    L = 4
+    f = family(L)
+    n = 3 * L * L // 2
+    U, dU, _ = orbital_derivatives(f.hamiltonian(THETA1), f.generators(THETA1), n)
+    dU5 = list(dU) + [np.asfortranarray(0.5 * dU[0] - dU[3])]
+    nw = 8
+    ku, kd = conditioned_configs(oracle_mod, U, nw, 1e6, seed=9)
+    with nat.DeviceSampler(np.array(f.nn, dtype=np.int32), U, U, nw, orbitals=orb(U, dU)) as ds:
+        ds.set_configs(ku, kd)
+        ds.set_orbital_derivs(dU, dU)
+        _, O4 = ds.measure_grad()
+        ds.set_orbital_derivs(dU5, dU5)
+        _, O5 = ds.measure_grad()
+    assert O4.shape == (nw, 4) and O5.shape == (nw, 5)
+    for w in range(nw):
+        ref = oracle_mod.log_derivs(U, dU5, ku[w]) + oracle_mod.log_derivs(U, dU5, kd[w])
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(O5[w] - ref).max() < 1e-9 * scale, (w, O5[w], ref)
+        assert np.abs(O4[w] - ref[:4]).max() < 1e-9 * scale, (w, O4[w], ref[:4])
+        assert abs(O5[w, 4] - (0.5 * O5[w, 0] - O5[w, 3])) < 1e-9 * scale
+
+
 @pytest.mark.parametrize("prec,delay", [(0, 1), (1, 4)])
 def test_measure_grad_after_sweeps(oracle_mod, prec, delay):
     """O_k after accepted moves (rank-1 updates, pending FP32 terms flushed by
