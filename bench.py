@@ -501,7 +501,8 @@ def run_grad(args):
     else:
         theta = THETA0.copy()
     gap = shell_gap(np.linalg.eigvalsh(fam.hamiltonian(theta)), n)
-    gs = GradientSampler(fam, theta, n, n, args.walkers, seed=args.seed, precision=nat.PREC_FP64)
+    gs = GradientSampler(fam, theta, n, n, args.walkers, seed=args.seed, precision=nat.PREC_FP64,
+                         delay=args.grad_delay)
     s = gs.sampler
     s.init_random(args.seed)
     for _ in range(args.warmup):     # warm-up steps do the timed steps' work (first launches included)
@@ -580,6 +581,8 @@ def main():
                     help="untimed steps after the timed region, each checking the carried W")
     ap.add_argument("--graph", action="store_true",
                     help="replay captured sweep sequences (no per-kernel times)")
+    ap.add_argument("--grad-delay", type=int, default=1,
+                    help="gradient form: pending FP64 rank-1 updates per flush (1: immediate)")
     ap.add_argument("--theta", default="seed", choices=("seed", "theta0"),
                     help="gradient form: parameters drawn with the seed, or theta0 = (0, 1, 0, 0)")
     ap.add_argument("--no-autotune", action="store_true",

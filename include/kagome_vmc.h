@@ -91,8 +91,9 @@ typedef struct kvmc_desc {
     int32_t kernel;         /* KVMC_KERNEL_* */
     int32_t refresh_policy; /* KVMC_REFRESH_* */
     int32_t refresh_k;      /* accepted moves per refresh for EVERY_K */
-    int32_t delay;          /* FP32 only: pending rank-1 updates kept before one
-                               rank-`delay` flush of W (<= 1: immediate) */
+    int32_t delay;          /* pending rank-1 updates kept before one rank-`delay`
+                               flush of W (<= 1: immediate, as the reference);
+                               FP32 and FP64 alike */
     int32_t threads;        /* threads per workgroup of the sweep kernel (0: auto) */
     uint64_t seed;          /* Philox key of the device RNG */
     int32_t orbitals;       /* KVMC_ORB_* */

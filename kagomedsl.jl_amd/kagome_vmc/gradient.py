@@ -150,7 +150,7 @@ class GradientSampler:
     complex handle."""
 
     def __init__(self, family: AnsatzFamily, theta, n_up: int, n_down: int, nwalkers: int,
-                 seed: int = 0, precision: int = nat.PREC_FP64):
+                 seed: int = 0, precision: int = nat.PREC_FP64, delay: int = 1):
         self.family = family
         self.theta = np.asarray(theta, dtype=float)
         H = family.hamiltonian(self.theta)
@@ -159,7 +159,7 @@ class GradientSampler:
         self.U_dn, self.dU_dn, _ = orbital_derivatives(H, G, n_down)
         cplx = _is_complex(self.U_up, self.U_dn, *self.dU_up, *self.dU_dn)
         self.sampler = nat.DeviceSampler(np.array(family.nn, dtype=np.int32), self.U_up, self.U_dn,
-                                         nwalkers, precision=precision,
+                                         nwalkers, precision=precision, delay=delay,
                                          kernel=nat.KERNEL_UNBIASED, seed=seed,
                                          orbitals=nat.ORB_COMPLEX if cplx else nat.ORB_REAL)
         self.sampler.set_orbital_derivs(self.dU_up, self.dU_dn)

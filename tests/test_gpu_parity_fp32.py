@@ -217,6 +217,38 @@ def test_fp64_streamed_w_tape_parity_L12(oracle_mod):
     assert cnt["refreshes"] == nref and nref > 0
 
 
+@pytest.mark.parametrize("L,delay", [(4, 4), (8, 8)])
+def test_fp64_delayed_updates_tape_parity(oracle_mod, L, delay):
+    """FP64 W with `delay` pending rank-1 terms (W resident in LDS at L=4,
+    streamed at L=8) against the oracle's immediate updates: the terms are
+    the same Sherman-Morrison factors, so decisions are identical except
+    within 1e-12 of a tie, and W and local energies agree to 1e-9."""
+    ham = system(L)
+    nw, n = 3, 300
+    ku, kd = well_conditioned_configs(oracle_mod, 83, nw, ham)
+    tu, tb = tape(nw, n, len(ham.nn), 41)
+    with sampler(ham, nw, delay=delay) as ds:
+        ds.set_configs(ku, kd)
+        cnt, out = ds.sweep(n, tu, tb, record=True)
+        e, _ = ds.measure()
+        kf = ds.get_configs()
+        W = [(ds.get_w(w, 0), ds.get_w(w, 1)) for w in range(nw)]
+    assert cnt["flushes"] > 0
+    for w in range(nw):
+        ch = oracle_mod.Chain.start(ham.nn, ham.U_up, ham.U_down, ku[w], kd[w])
+        oc, _, _ = oracle_mod.run_chain(ch, n, tape_u=tu[w], tape_bond=tb[w])
+        if not np.array_equal(out[w], oc):
+            t, m, _ = first_divergence_is_near_tie(oracle_mod, ham, ku[w], kd[w], tu[w], tb[w], out[w], oc)
+            assert m < 1e-12, f"walker {w}: decision differs at sweep {t} with margin {m:.3e}"
+            continue
+        np.testing.assert_array_equal(kf[0][w], ch.kup)
+        np.testing.assert_array_equal(kf[1][w], ch.kdn)
+        assert _max_rel(W[w][0], ch.Wup.real) < 1e-9
+        assert _max_rel(W[w][1], ch.Wdn.real) < 1e-9
+        e_or = oracle_mod.getol(ham.nn, ch.kup, ch.kdn, ch.Wup, ch.Wdn)
+        assert abs(e[w] - e_or) < 1e-9 * max(1.0, abs(e_or))
+
+
 def test_mixed_groups_follow_oracle_chains(oracle_mod):
     """The mixed-size batch (L = 4, 6, 8, 10, 12 groups, FP32 W with delay 12,
     FP64 refresh every 64 accepts; W in LDS for L <= 6, streamed above):
