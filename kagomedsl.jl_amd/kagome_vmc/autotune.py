@@ -4,7 +4,7 @@ Two knobs set how the sweep kernel maps onto the GPU:
   * threads -- workgroup size (one workgroup per walker): more threads
                stream W faster in a flush, fewer let more walkers share a
                CU and hide the latency of the proposals;
-  * delay   -- FP32 only: accepted moves kept as pending rank-1 terms before
+  * delay   -- accepted moves kept as pending rank-1 terms before
                W is streamed (fewer HBM passes, more LDS and register use).
 The best pair depends on the cluster size, the precision and the walker
 count, so it is measured: each candidate runs a short timed stretch of the
@@ -30,7 +30,7 @@ def candidates(ns: int, precision: int):
     nvec = ns // vec if ns % vec == 0 else ns
     floor = (nvec + 63) // 64 * 64
     threads = sorted({max(t, floor) for t in (64, 128, 256) if max(t, floor) <= 512})
-    delays = (4, 8, 12) if precision == nat.PREC_FP32 else (1,)
+    delays = (4, 8, 12) if precision == nat.PREC_FP32 else (1, 4, 8)
     return [(t, d) for t in threads for d in delays]
 
 

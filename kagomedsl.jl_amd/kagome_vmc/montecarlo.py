@@ -56,7 +56,7 @@ class MC:
     link_in, link_inter`` (optional).  Device keys: ``nwalkers`` (1),
     ``precision`` ("fp64" | "fp32"), ``kernel`` ("reference" | "unbiased"),
     ``refresh`` ("reference" or an int k: refresh every k accepted moves),
-    ``delay`` (pending rank-1 updates per flush, fp32 only), ``threads``
+    ``delay`` (pending rank-1 updates per flush; 1 = immediate), ``threads``
     (either may be "auto": chosen by the autotuner), ``seed`` (device RNG
     key; defaults to the context seed)."""
 

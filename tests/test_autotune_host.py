@@ -11,7 +11,7 @@ def test_candidates_respect_the_column_floor():
     c = candidates(432, nat.PREC_FP32)
     assert sorted({t for t, _ in c}) == [128, 256] and sorted({d for _, d in c}) == [4, 8, 12]
     # FP64 has no delayed updates; 216 double2 rows -> 256 threads
-    assert candidates(432, nat.PREC_FP64) == [(256, 1)]
+    assert candidates(432, nat.PREC_FP64) == [(256, 1), (256, 4), (256, 8)]
     # small cluster: one wavefront is a candidate
     assert (64, 8) in candidates(48, nat.PREC_FP32)
 
