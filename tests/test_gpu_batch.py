@@ -35,12 +35,13 @@ def test_groups_match_their_solo_runs():
         assert res[g][0]["sweeps"] == 3 * nw * ham.N_up
 
 
-def test_autotune_on_device(tmp_path):
+@pytest.mark.parametrize("prec", [nat.PREC_FP32, nat.PREC_FP64])
+def test_autotune_on_device(tmp_path, prec):
     from kagome_vmc.autotune import autotune, candidates
     ham = pi_flux_cluster(4)
-    r = autotune(ham, 128, nat.PREC_FP32, 32, steps=1, cache_path=str(tmp_path / "t.json"))
-    assert (r["threads"], r["delay"]) in candidates(ham.ns, nat.PREC_FP32)
-    assert r["walker_sweeps_per_s"] > 0 and len(r["trials"]) == len(candidates(ham.ns, nat.PREC_FP32))
+    r = autotune(ham, 128, prec, 32, steps=1, cache_path=str(tmp_path / "t.json"))
+    assert (r["threads"], r["delay"]) in candidates(ham.ns, prec)
+    assert r["walker_sweeps_per_s"] > 0 and len(r["trials"]) == len(candidates(ham.ns, prec))
 
 
 def test_group_graph_matches_per_handle_sweeps():
