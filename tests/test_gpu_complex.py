@@ -125,6 +125,37 @@ def test_complex_fp64_tape_chain_parity(oracle_mod, L, nsweeps, policy):
     assert cnt["refreshes"] == nref
 
 
+@pytest.mark.parametrize("L,delay", [(4, 4), (8, 8)])
+def test_complex_fp64_delayed_tape_parity(oracle_mod, L, delay):
+    """FP64 complex W with `delay` pending complex rank-1 terms against the
+    oracle's immediate complex updates: identical decisions (ties within
+    1e-12 excepted), W and the local energy to 1e-9."""
+    ham = complex_system(L)
+    nw, nsweeps = 3, 300
+    ku, kd = conditioned(oracle_mod, ham, nw, 23)
+    tu, tb = tape(nw, nsweeps, len(ham.nn), 5)
+    with sampler(ham, nw, delay=delay) as ds:
+        ds.set_configs(ku, kd)
+        cnt, out = ds.sweep(nsweeps, tu, tb, record=True)
+        kf = ds.get_configs()
+        e, _ = ds.measure()
+        W = [(ds.get_w(w, 0), ds.get_w(w, 1)) for w in range(nw)]
+    assert cnt["flushes"] > 0
+    for w in range(nw):
+        ch = oracle_mod.Chain.start(ham.nn, ham.U_up, ham.U_down, ku[w], kd[w])
+        oc, _, _ = oracle_mod.run_chain(ch, nsweeps, tape_u=tu[w], tape_bond=tb[w])
+        if not np.array_equal(out[w], oc):
+            t, m, _ = first_divergence_is_near_tie(oracle_mod, ham, ku[w], kd[w], tu[w], tb[w], out[w], oc)
+            assert m < 1e-12, f"walker {w}: decision differs at sweep {t} with margin {m:.3e}"
+            continue
+        np.testing.assert_array_equal(kf[0][w], ch.kup)
+        np.testing.assert_array_equal(kf[1][w], ch.kdn)
+        assert _max_rel(W[w][0], ch.Wup) < 1e-9
+        assert _max_rel(W[w][1], ch.Wdn) < 1e-9
+        e_or = oracle_mod.getol(ham.nn, ch.kup, ch.kdn, ch.Wup, ch.Wdn)
+        assert abs(e[w] - e_or) < 1e-9 * max(1.0, abs(e_or))
+
+
 @pytest.mark.parametrize("L,delay", [(4, 12), (8, 12), (8, 4)])
 def test_complex_fp32_tape_parity(oracle_mod, L, delay):
     """FP32 complex W (W in LDS at L=4, streamed from HBM at L=8) with
